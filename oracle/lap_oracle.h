@@ -1,0 +1,104 @@
+/*
+ * lap_oracle.h -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * CPU restatement of the reference LAP solver's serial path, used only as the
+ * checker by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+ * leg.  Nothing in lap_amd/ links, loads or calls this library.
+ *
+ * Every function follows one reference function operation for operation
+ * (same loop order, same association of floating-point operations), so that
+ * built with -ffp-contract=off it is bitwise equal to the reference's
+ * simple_blas build.  The citation next to each declaration names the
+ * reference lines it restates.  Pinning status of each function is listed in
+ * lap_oracle.c's header.
+ */
+#ifndef LAP_ORACLE_H
+#define LAP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- BLAS-1 (simple_blas.c:4-22) ------------------------------------- */
+double orc_dot(int n, const double *v, const double *w);            /* simple_blas.c:4-10  */
+void   orc_scal(int n, double alpha, double *v);                    /* simple_blas.c:12-16 */
+void   orc_axpy(int n, double alpha, const double *x, double *y);   /* simple_blas.c:18-22 */
+
+/* ---- sparse kernels ---------------------------------------------------- */
+/* y = alpha*A*x + beta*y, row-sequential (simple_blas.c:24-44) */
+void orc_csr_matvec(int n, const int *ia, const int *ja, const double *a,
+                    const double *x, double *y, double alpha, double beta);
+
+/* Triangular solves (simple_blas.c:46-85).  The diagonal of row i is the
+ * entry whose column equals i when the row holds one (the GPU driver's
+ * augmented L+D / D+U, cg_driver2.c:95-144, and the IC0 factors,
+ * simple_blas.c:184-223); otherwise it is diag[i] (the strict L/U of the CPU
+ * path).  Off-diagonal terms are subtracted in CSR order. */
+void orc_lower_solve(int n, const int *ia, const int *ja, const double *a,
+                     const double *diag, const double *x, double *y);
+void orc_upper_solve(int n, const int *ia, const int *ja, const double *a,
+                     const double *diag, const double *x, double *y);
+
+/* element-wise (simple_blas.c:91-136) */
+void orc_vec_vec(int n, const double *x, const double *y, double *res);
+void orc_reciprocal(int n, const double *v, double *res);
+void orc_sqrt(int n, const double *v, double *res);
+
+/* IC0 (simple_blas.c:139-182): right-looking factorisation in place on the
+ * upper factor U = L^T (rows sorted, diagonal first), then the transpose is
+ * scattered into la (L pattern, rows sorted, diagonal last). */
+void orc_ichol_factor(int n, const int *uia, const int *uja, double *ua,
+                      const int *lia, const int *lja, double *la);
+/* IC0 apply y = U^{-1} L^{-1} x (simple_blas.c:184-223); aux has n entries */
+void orc_ichol_apply(int n, const int *lia, const int *lja, const double *la,
+                     const int *uia, const int *uja, const double *ua,
+                     const double *x, double *y, double *aux);
+
+/* ---- preconditioners + PCG -------------------------------------------- */
+enum orc_prec_kind {
+  ORC_PREC_NONE = 0,
+  ORC_PREC_LINE_JACOBI = 1,   /* line_jacobi.c:3-7 */
+  ORC_PREC_IT_JACOBI = 2,     /* it_jacobi.c:4-32  */
+  ORC_PREC_GS_STD = 3,        /* GS.c:5-42         */
+  ORC_PREC_GS_IT = 4,         /* GS.c:45-92        */
+  ORC_PREC_GS_IT2 = 5,        /* GS.c:95-159       */
+  ORC_PREC_ICHOL = 6          /* simple_blas.c:184-223 */
+};
+
+/* Preconditioner operands, the oracle's counterpart of the reference's
+ * pdata (common.h:29-50). */
+typedef struct {
+  int kind;
+  int n, m, k;
+  const int *lia, *lja; const double *la; /* strict L (CPU path) or L factor */
+  const int *uia, *uja; const double *ua; /* strict U (CPU path) or U factor */
+  const double *d, *d_r;                  /* diag(A) and 1/diag(A)          */
+  double *aux1, *aux2, *aux3;             /* n-vectors of scratch            */
+} orc_prec;
+
+void orc_prec_apply(const int *ia, const int *ja, const double *a,
+                    const orc_prec *p, const double *x, double *y);
+
+/* PCG (cg.c:5-140, CPU branch).  hist needs maxit+2 entries.
+ * quiet != 0 suppresses the per-iteration log lines. */
+void orc_cg(int n, const int *ia, const int *ja, const double *a,
+            double *x, const double *b, double tol, const orc_prec *p,
+            int maxit, int *it, int *flag, double *hist, int quiet);
+
+/* ---- OpenMP-structured variants (openmp_blas.c:21-52, :166-176) ------- */
+/* Timed CPU baseline only: SpMV row-parallel on spmv_threads threads (the
+ * reference hard-codes 16, openmp_blas.c:37), dot a parallel reduction on
+ * dot_threads threads, every other vector op sequential.  Numerically these
+ * differ from orc_cg only through the order of the dot-product sums. */
+void orc_omp_cg(int n, const int *ia, const int *ja, const double *a,
+                double *x, const double *b, double tol, const orc_prec *p,
+                int maxit, int *it, int *flag, double *hist,
+                int spmv_threads, int dot_threads);
+void orc_omp_csr_matvec(int n, const int *ia, const int *ja, const double *a,
+                        const double *x, double *y, double alpha, double beta,
+                        int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
