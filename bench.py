@@ -1,0 +1,183 @@
+"""Benchmark: seconds to relative residual 1e-10 and HBM GB/s for FP64 PCG on
+the 3D 7-point graph Laplacian 256^3 + 1e-8 I (n = 16,777,216, nnz =
+117,047,296), b ~ N(0,1) seeded and mean-subtracted, Jacobi preconditioner.
+
+One step = one complete PCG solve (x0 = 0 to ||r|| < 1e-10 ||r0||) with the
+matrix, preconditioner and right-hand side already resident in HBM.  Prints
+one JSON line.  Single GPU only.
+
+    python bench.py [--steps K] [--warmup W] [--size 256] [--prec line_jacobi]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n: int, nnz: int) -> int:
+    """Algorithmic bytes of one FP64 CSR SpMV with beta = 0:
+    12 B per nonzero (int32 column + FP64 value), 4 B per row pointer,
+    8 B to read x and 8 B to write y per row."""
+    return 12 * nnz + 4 * (n + 1) + 16 * n
+
+
+def jacobi_iteration_bytes(n: int, nnz: int) -> int:
+    """SpMV + 88 B per row of vector traffic (x, r, p, q, d_r reads; x, r
+    writes; p update)."""
+    return spmv_bytes(n, nnz) + 88 * n
+
+
+def cpu_baseline(Ah, b, prec: str, tol: float, gpu_iters: int, sample_iters: int):
+    """The reference's OpenMP cg() (compiled from its sources into
+    oracle/_ref/) timed on this host for `sample_iters` iterations of the same
+    system; seconds to the GPU's iteration count are extrapolated from the
+    per-iteration time.  Falls back to the oracle's OpenMP restatement."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    maxit = max(sample_iters - 1, 1)
+    if os.path.exists(oracle.REF_OMP):
+        ref = oracle.Reference(oracle.REF_OMP)
+        pdk = ref.make_pdata(Ah, prec)
+        t0 = time.perf_counter()
+        _, it, _, _ = ref.cg(Ah, b, prec, tol=tol, maxit=maxit, pdk=pdk)
+        dt = time.perf_counter() - t0
+        kind = "reference"
+        what = ("the reference's OpenMP cg() (openmp_blas.c build: SpMV on its hard-coded 16 "
+                f"threads, dot on OMP_NUM_THREADS={threads}, other vector ops serial)")
+    else:
+        orc = oracle.Oracle()
+        t0 = time.perf_counter()
+        _, it, _, _ = orc.cg(Ah, b, prec, tol=tol, maxit=maxit, omp=(16, threads))
+        dt = time.perf_counter() - t0
+        kind = "port"
+        what = "the oracle's OpenMP restatement of the reference CPU path"
+    per_iter = dt / it
+    return {
+        "value": per_iter * gpu_iters,
+        "unit": "s",
+        "cores": max(16, threads),
+        "kind": kind,
+        "sample": (f"{it} PCG iterations of {what} on the same 256^3 system "
+                   f"({dt:.2f} s, {per_iter * 1e3:.1f} ms/iteration), scaled to the "
+                   f"GPU's {gpu_iters} iterations"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--prec", default="line_jacobi")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the solve would shard)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from lap_amd import solver as S
+    from lap_amd.problems import grid_laplacian, rhs_normal
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    Ah = grid_laplacian([args.size] * 3, 1e-8)
+    b_h = rhs_normal(Ah.n, 12345)
+    log(f"built {args.size}^3 Laplacian n={Ah.n} nnz={Ah.nnz} in {time.perf_counter() - t0:.1f}s")
+    A = S.DeviceCSR.from_csr(Ah)
+    b = S.to_dev(b_h)
+    P = S.make_preconditioner(A, Ah, args.prec)
+    log("setup done")
+
+    for w in range(args.warmup):
+        r = S.pcg(A, b, P, tol=args.tol, maxit=25000)
+        log(f"warmup {w}: {r.iterations} iterations, {r.gpu_ms:.1f} ms, flag {r.flag}")
+    torch.cuda.synchronize()
+    results = []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        results.append(S.pcg(A, b, P, tol=args.tol, maxit=25000))
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t_start) / max(args.steps, 1)
+    its = results[-1].iterations
+    gpu_ms = float(np.mean([r.gpu_ms for r in results]))
+    flags = {r.flag for r in results}
+    log(f"timed {args.steps} solves: {its} iterations, wall {wall * 1e3:.1f} ms, "
+        f"device {gpu_ms:.1f} ms, flags {flags}")
+
+    # dominant kernel: the fused SpMV q = A p (+ p.q), timed live with HIP
+    # events on the library stream
+    x = torch.randn(A.n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    spmv_ms = S.spmv_timed(A, x, y, args.spmv_reps, fused=True)
+    bytes_spmv = spmv_bytes(A.n, A.nnz)
+    achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9
+    it_bytes = jacobi_iteration_bytes(A.n, A.nnz)
+    log(f"spmv (fused) {spmv_ms:.4f} ms -> {achieved:.0f} GB/s")
+
+    out = {
+        "metric": "s to rel-res 1e-10 + HBM GB/s, FP64 PCG vs mixed-prec PCG, 256^3 Laplacian",
+        "value": wall,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"3D 7-point graph Laplacian {args.size}^3 + 1e-8*I, b~N(0,1) seed "
+                         f"12345 mean-subtracted, FP64 PCG ({args.prec}) from x0=0 to "
+                         f"rel-res {args.tol:g}"),
+            "n": A.n, "nnz": A.nnz, "preconditioner": args.prec,
+            "iterations": its, "flag": sorted(flags),
+        },
+        "device_ms_per_solve": gpu_ms,
+        "iteration_gbs": it_bytes * its / (gpu_ms * 1e-3) / 1e9,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_spmv_pq (fused CSR SpMV q=A p with p.q)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": bytes_spmv,
+            "avg_ms": spmv_ms,
+        },
+    }
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(Ah, b_h, args.prec, args.tol, its, args.cpu_iters)
+        log(f"cpu baseline in {time.perf_counter() - t0:.1f}s: {out['cpu_baseline']['value']:.1f} s")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,233 @@
+/*
+ * lap_gfx950.h -- C ABI of the MI355X (gfx950) backend for the LAP PCG solver.
+ *
+ * liblapgfx.so exports three groups of symbols:
+ *
+ *  1. The reference's HIP backend surface (hip_blas.h + devMem.h), symbol for
+ *     symbol, so the reference's host code (cg.c, prec.c, GS.c, it_jacobi.c,
+ *     line_jacobi.c, blas.c, io_utils.c, cg_driver*.c, mm_driver.c built with
+ *     -DHIP=1) links against this library in place of hip_blas.o + devMem.o +
+ *     rocSPARSE + rocBLAS.
+ *  2. The reference's solver layer (common.h:52-74, blas.h:3-48): cg(),
+ *     prec_function(), the preconditioners and the blas.h wrappers.  Linking
+ *     these from the library instead of the reference's objects switches the
+ *     solve to the fused, device-resident implementation.  A definition in the
+ *     executable takes precedence over the library's (ELF interposition), so
+ *     both link modes use the same library.
+ *  3. lap_* extensions with no reference counterpart (status reporting, the
+ *     explicit solve entry points, mixed precision, block and batched PCG,
+ *     adjoint gradients).
+ *
+ * Conventions (as the reference): every pointer argument of groups 1-2 is a
+ * device pointer allocated with mallocForDevice unless stated otherwise;
+ * scalars alpha/beta of hip_csr_matvec are host pointers; hip_dot returns a
+ * host value; every call is synchronous at return; the library keeps
+ * per-process state (analysis plans) and is not thread-safe.  Errors are
+ * printed to stderr and recorded; lap_last_status()/lap_last_error() return
+ * them (the reference's functions return void).
+ */
+#ifndef LAP_GFX950_H
+#define LAP_GFX950_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Preconditioner state.  Same member order and types as the reference's
+ * pdata (common.h:29-50, with real_type = double), so a pdata* from the
+ * reference's drivers can be passed where a lap_pdata* is expected. */
+typedef struct lap_pdata {
+  int *lia;
+  int *lja;
+  double *la;
+  int lnnz;
+  int *uia;
+  int *uja;
+  double *ua;
+  int unnz;
+  double *ichol_vals;
+  double *d;
+  double *d_r;
+  int n;
+  double *aux_vec1, *aux_vec2, *aux_vec3;
+  char *prec_op;
+  int m, k;
+} lap_pdata;
+
+/* ===================== 1. hip_blas.h / devMem.h surface ================== */
+
+/* hip_blas.h:5 (hip_blas.cpp:23-44): creates the library's stream and
+ * device workspace.  Optional: every entry point initialises lazily. */
+void initialize_handles(void);
+
+/* hip_blas.h:7-14 (hip_blas.cpp:46-100).  Builds the SpMV plan (row blocks
+ * binned by row length) for the CSR arrays (ia, ja); option "A", "L" or "U"
+ * is accepted for compatibility.  Plans are keyed by (ia, ja, n, nnz), so
+ * hip_csr_matvec also works on matrices that were never analysed. */
+void analyze_spmv(const int n, const int nnz, int *ia, int *ja, double *a,
+                  const double *x, double *result, char *option);
+
+/* hip_blas.h:16-24 (hip_blas.cpp:102-173).  Level analysis of L+D (diagonal
+ * last in each row) and D+U (diagonal first) for the triangular solves. */
+void initialize_and_analyze_L_and_U_solve(const int n, const int nnzL, int *lia, int *lja,
+                                          double *la, const int nnzU, int *uia, int *uja,
+                                          double *ua);
+
+/* hip_blas.h:26 (hip_blas.cpp:432-445): returns v.w on the host. */
+double hip_dot(const int n, const double *v, const double *w);
+/* hip_blas.h:28 (hip_blas.cpp:447-452): v *= alpha. */
+void hip_scal(const int n, const double alpha, double *v);
+/* hip_blas.h:30 (hip_blas.cpp:454-465): y += alpha x. */
+void hip_axpy(const int n, const double alpha, const double *x, double *y);
+
+/* hip_blas.h:32-41 (hip_blas.cpp:467-532): result = al*A*x + bet*result,
+ * al/bet host pointers.  Rows are summed in CSR order.  bet == 0 does not
+ * read result (BLAS convention; the CPU path computes result*0). */
+void hip_csr_matvec(const int n, const int nnz, const int *ia, const int *ja, const double *a,
+                    const double *x, double *result, const double *al, const double *bet,
+                    const char *kind);
+
+/* hip_blas.h:43-59 (hip_blas.cpp:534-591): result = T^{-1} x for a lower
+ * (upper) triangular CSR T.  The pivot of row i is the entry with column i
+ * when the row holds one (the GPU driver's L+D / D+U), else diagonal[i]
+ * (the CPU path's strict L / U); off-diagonal terms are subtracted in CSR
+ * order, as simple_blas.c:46-85. */
+void hip_lower_triangular_solve(const int n, const int nnzL, const int *lia, const int *lja,
+                                const double *la, const double *diagonal, const double *x,
+                                double *result);
+void hip_upper_triangular_solve(const int n, const int nnzU, const int *uia, const int *uja,
+                                const double *ua, const double *diagonal, const double *x,
+                                double *result);
+
+/* hip_blas.h:61-71 (hip_blas.cpp:368-430, :597-630): element-wise ops.
+ * reciprocal gives 0 where v == 0; sqrt gives 0 where v <= 0. */
+void hip_vec_vec(const int n, const double *x, const double *y, double *res);
+void hip_vector_reciprocal(const int n, const double *v, double *res);
+void hip_vector_sqrt(const int n, const double *v, double *res);
+void hip_vec_copy(const int n, const double *src, double *dest);
+void hip_vec_zero(const int n, double *vec);
+void hip_vec_set(const int n, double value, double *vec);
+
+/* hip_blas.h:73-77 (hip_blas.cpp:175-317): IC(0) factorisation of the SPD
+ * matrix (ia, ja, a) on the device, level-scheduled, written in place into
+ * the lower triangle (diagonal included) of a; the upper triangle of a is
+ * left unchanged.  The factor equals the reference CPU initialize_ichol's
+ * (simple_blas.c:139-182) bitwise. */
+void initialize_ichol(const int n, const int nnzA, int *ia, int *ja, double *a);
+
+/* hip_blas.h:79-85 (hip_blas.cpp:319-366): y = L^{-T} L^{-1} x with the
+ * factor set up by initialize_ichol on prec_data->ichol_vals; the
+ * arithmetic order is simple_ichol's (simple_blas.c:184-223). */
+void hip_ichol(const int *ia, const int *ja, double *a, const int nnzA, lap_pdata *prec_data,
+               double *x, double *y);
+
+/* devMem.h:2-4 (devMem.cpp:9-48).  memcpyDevice's type is "H2D", "D2H" or
+ * "D2D" (D2D copies device to device; the reference passes
+ * DeviceToHost there, devMem.cpp:33-39). */
+void *mallocForDevice(void *x, int n, int size);
+void memcpyDevice(void *dest, void *src, int n, int size, char *type);
+void freeDevice(void *p);
+
+/* ===================== 2. solver layer (common.h, blas.h) ================ */
+
+/* common.h:54-66 (cg.c:5-140): PCG from x0 = x.  Device-resident: the
+ * preconditioner, SpMV, dots and updates stay on the GPU, alpha/beta and
+ * the convergence test are computed on the device, iterations are replayed
+ * from a HIP graph and the host polls for convergence.  The per-iteration
+ * "CG: it %d, res norm %5.5e" lines and the "TRUE Norm of r" line are
+ * printed after the solve from the device residual history.
+ * res_norm_history is a host array of at least maxit+2 entries; *flag is
+ * 0 converged, 1 maxit reached, 2 failure (non-finite value or breakdown).
+ * Recognised prec_op: none, line_jacobi, it_jacobi, GS_std, GS_it, GS_it2,
+ * ichol. */
+void cg(int n, double nnz, int *ia, int *ja, double *a, double *x, double *b, double tol,
+        lap_pdata *prec_data, int maxit, int *it, int *flag, double *res_norm_history);
+
+/* common.h:52 (prec.c:6-45) and common.h:70-74 */
+void prec_function(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, double *x,
+                   double *y);
+void GS_std(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, double *vec_in,
+            double *vec_out);
+void GS_it(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, double *vec_in,
+           double *vec_out);
+void GS_it2(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, double *vec_in,
+            double *vec_out);
+void it_jacobi(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, double *vec_in,
+               double *vec_out);
+void line_jacobi(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, double *vec_in,
+                 double *vec_out);
+
+/* blas.h:3-48 (blas.c:15-186 with -DHIP=1): forward to the hip_* above. */
+double dot(const int n, const double *v, const double *w);
+void axpy(const int n, const double alpha, double *x, double *y);
+void scal(const int n, const double alpha, double *v);
+void csr_matvec(const int n, const int nnz, const int *ia, const int *ja, const double *a,
+                const double *x, double *result, const double *al, const double *bet,
+                const char *kind);
+void lower_triangular_solve(const int n, const int nnz, const int *lia, const int *lja,
+                            const double *la, const double *diag, const double *x,
+                            double *result);
+void upper_triangular_solve(const int n, const int nnz, const int *uia, const int *uja,
+                            const double *ua, const double *diag, const double *x,
+                            double *result);
+void ichol(const int *ia, const int *ja, double *a, const int nnzA, lap_pdata *prec_data,
+           double *x, double *y);
+void vec_vec(const int n, const double *x, double *y, double *res);
+void vector_reciprocal(const int n, const double *v, double *res);
+void vector_sqrt(const int n, const double *v, double *res);
+void vec_copy(const int n, double *src, double *dest);
+void vec_zero(const int n, double *vec);
+
+/* ===================== 3. lap_* extensions =============================== */
+/* No reference counterpart. */
+
+#define LAP_OK 0
+#define LAP_ERR_ARG 1      /* invalid argument or sizes                   */
+#define LAP_ERR_HIP 2      /* a HIP runtime call failed                   */
+#define LAP_ERR_STRUCT 3   /* matrix structure not as required            */
+#define LAP_ERR_NUMERIC 4  /* zero/negative pivot, non-finite value        */
+#define LAP_ERR_PREC 5     /* unknown preconditioner name                 */
+
+int lap_last_status(void);
+const char *lap_last_error(void);
+void lap_clear_error(void);
+
+/* Drops every cached analysis plan, factor and work buffer. */
+void lap_release_all(void);
+
+/* Result of one PCG solve. */
+typedef struct lap_solve_info {
+  int iterations;    /* as *it of cg()                                     */
+  int flag;          /* 0 converged, 1 maxit, 2 failure                     */
+  double res0;       /* ||b - A x0||                                        */
+  double res_final;  /* recursive residual norm at exit                     */
+  double true_res;   /* ||b - A x|| recomputed at exit                      */
+  double gpu_ms;     /* device time of the iteration loop (HIP events)      */
+} lap_solve_info;
+
+/* cg() without printing: the same solve, explicit result.  history (host)
+ * may be NULL or hold maxit+2 entries.  Returns a LAP_* status. */
+int lap_pcg(int n, int nnz, const int *ia, const int *ja, const double *a, double *x,
+            const double *b, double tol, lap_pdata *prec_data, int maxit, double *history,
+            lap_solve_info *info);
+
+/* Kernel-level entry points used by benchmarks and tests (device pointers):
+ * y = A x with the fused SpMV, launched `reps` times back to back on the
+ * library stream; returns the average device milliseconds per launch. */
+double lap_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
+                      const double *x, double *y, int reps);
+/* Same for the PCG's fused kernel q = A p with p.q and alpha in its
+ * epilogue (the dominant kernel of a Jacobi PCG iteration). */
+double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
+                          const double *p, double *q, int reps);
+
+/* Library configuration knobs (for tuning sweeps): name/value pairs.
+ * "spmv_rows" (rows per SpMV block), "spmv_xcd" (1 = XCD-aware block
+ * order), "graph_iters" (iterations per HIP graph launch).  Returns the
+ * previous value, or -1 for an unknown name. */
+int lap_set_option(const char *name, int value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

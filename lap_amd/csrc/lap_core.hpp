@@ -1,0 +1,220 @@
+// lap_core.hpp -- internal definitions shared by the gfx950 backend.
+//
+// Execution model: one HIP stream per process (a blocking stream at default
+// priority, so it orders with work the caller put on the legacy default
+// stream), 64-lane wavefronts, 256-thread workgroups unless stated.  Every
+// public entry point leaves the stream synchronised on return.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/lap_gfx950.h"
+
+namespace lap {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------ errors
+void set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+bool hip_ok(hipError_t e, const char *what, const char *file, int line);
+#define LAP_HIP(expr) ::lap::hip_ok((expr), #expr, __FILE__, __LINE__)
+
+// ------------------------------------------------------------------ runtime
+hipStream_t stream();
+int device_cus();
+void sync();  // hipStreamSynchronize(stream()) with error capture
+// True when p is device (or managed) memory of the HIP runtime.  Used to
+// refuse host pointers before any kernel could dereference them.
+bool is_device_ptr(const void *p);
+
+// Device buffer owned by the library (RAII over hipMalloc/hipFree).
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b);
+  ~DevBuf();
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+  void upload(const void *src, size_t b, size_t off = 0);
+};
+using DevBufPtr = std::shared_ptr<DevBuf>;
+DevBufPtr dev_alloc(size_t bytes);
+template <class T>
+DevBufPtr dev_upload(const std::vector<T> &v) {
+  auto b = dev_alloc(v.size() * sizeof(T) + 16);
+  if (!v.empty()) b->upload(v.data(), v.size() * sizeof(T));
+  return b;
+}
+template <class T>
+std::vector<T> dev_download(const T *dptr, size_t count) {
+  std::vector<T> h(count);
+  if (count) {
+    LAP_HIP(hipMemcpyAsync(h.data(), dptr, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+    sync();
+  }
+  return h;
+}
+
+// Cached n-vectors reused across solves (avoids hipMalloc in timed regions).
+double *scratch_vec(const char *tag, size_t n);
+
+// ------------------------------------------------------------ reductions
+// Fence-free deterministic reductions.  A producer kernel writes one partial
+// sum per workgroup with a plain store; the consumer kernel (the next launch
+// on the stream, so the kernel boundary makes the partials visible) sums the
+// partials in a fixed order.  When the consumer is a full grid, every
+// workgroup performs the same sum redundantly and gets the same bits.  This
+// avoids agent-scope fences inside kernels, which on gfx950 write back and
+// invalidate the XCD's L2 (buffer_wbl2 / buffer_inv) once per workgroup.
+constexpr int kMaxPartialBlocks = 8192;
+struct ReduceWs {
+  double *partials;  // [kMaxPartialBlocks * 4]
+  double *scalars;   // [64] general device scalars
+};
+ReduceWs &reduce_ws();
+
+// -------------------------------------------------------- device helpers
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Sum over a block of BS threads; the result is valid in thread 0.
+// Fixed summation order, so deterministic.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < BS / kWave; ++i) r += lds[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Sum of partials[0..cnt) (stride `stride` doubles) in a fixed order,
+// returned to every thread of the block.
+template <int BS>
+__device__ __forceinline__ double sum_partials(const double *partials, int cnt, int stride,
+                                               double *lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < cnt; i += BS) s += partials[(size_t)i * stride];
+  s = block_sum<BS>(s, lds);
+  if (threadIdx.x == 0) lds[0] = s;
+  __syncthreads();
+  const double r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+// Work distribution of a persistent grid over nblk logical blocks.
+// Workgroups are dispatched round-robin over the 8 XCDs (XCD = id mod 8);
+// with `xcd` set, the workgroups of XCD k walk the k-th contiguous eighth of
+// the logical blocks, so neighbouring rows (which share x entries) are
+// processed under one L2.  Every logical block is visited exactly once.
+struct BlockWalk {
+  int begin, end, step;
+};
+__device__ __forceinline__ BlockWalk block_walk(int nblk, int xcd) {
+  const int g = gridDim.x, id = blockIdx.x;
+  if (!xcd || g < 8) return BlockWalk{id, nblk, g};
+  const int k = id % 8, lw = id / 8;
+  const int nw = g / 8 + (k < g % 8 ? 1 : 0);
+  const int lo = (int)(((long long)nblk * k) / 8), hi = (int)(((long long)nblk * (k + 1)) / 8);
+  return BlockWalk{lo + lw, hi, nw};
+}
+
+// -------------------------------------------------------------- options
+struct Options {
+  int spmv_rows = 256;   // rows per SpMV block (<= kBlock * 2)
+  int spmv_xcd = 0;      // XCD-aware block order
+  int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
+  int graph_iters = 16;  // PCG iterations per graph launch
+  int use_graph = 1;
+};
+Options &options();
+
+inline unsigned grid_for(size_t n, int per_thread = 4, int bs = kBlock, unsigned cap = 0) {
+  size_t g = (n + (size_t)bs * per_thread - 1) / ((size_t)bs * per_thread);
+  if (g < 1) g = 1;
+  if (cap && g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// ------------------------------------------------------------- SpMV plan
+struct SpmvPlan {
+  int n = 0, nnz = 0;
+  int nblocks = 0;        // row blocks
+  DevBufPtr rowblk;       // int[nblocks + 1]
+  int max_row = 0;
+};
+// Finds or builds the plan for (ia, ja, n, nnz) (device arrays).
+const SpmvPlan *spmv_plan(int n, int nnz, const int *ia, const int *ja);
+
+// y = alpha*A*x + beta*y on the library stream (no sync).
+void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *a,
+                 const double *x, double *y, double alpha, double beta,
+                 const int *done = nullptr);
+// Persistent SpMV grid: min(row blocks, 8 workgroups per CU).
+unsigned spmv_grid(const SpmvPlan &p);
+
+// --------------------------------------------------- triangular solve plan
+struct TriPlan {
+  int n = 0, nnz = 0;
+  bool lower = true;
+  int nlevels = 0;
+  std::vector<int> level_ptr;  // host, nlevels + 1
+  DevBufPtr order;             // int[n]: rows sorted by level
+  int max_level_rows = 0;
+  bool has_diag = true;        // every row holds its pivot
+};
+const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower);
+// y = T^{-1} x (level-scheduled); ctl_done (device int*, may be null) makes
+// every launch a no-op once *ctl_done != 0.
+void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
+                      const double *diag, const double *x, double *y, const int *ctl_done);
+
+// -------------------------------------------------------------- IC0 plan
+struct IcPlan {
+  int n = 0;
+  int nnzL = 0;
+  DevBufPtr lia, lja, lval;  // L (diagonal last)
+  DevBufPtr uia, uja, uval;  // U = L^T (diagonal first)
+  const TriPlan *lplan = nullptr, *uplan = nullptr;
+  bool ok = true;
+};
+const IcPlan *ic_plan_find(const double *vals);
+const IcPlan *ic_factor(int n, int nnzA, const int *ia, const int *ja, double *a);
+void ic_apply_launch(const IcPlan &p, const double *x, double *y, double *aux,
+                     const int *ctl_done);
+
+// ---------------------------------------------------- elementwise launches
+// `done` (device int*, may be null): the launch is a no-op once *done != 0.
+void launch_axpy(int n, double alpha, const double *x, double *y, const int *done = nullptr);
+void launch_scal(int n, double alpha, double *v, const int *done = nullptr);
+void launch_vec_vec(int n, const double *x, const double *y, double *res,
+                    const int *done = nullptr);
+void launch_copy(int n, const double *src, double *dst, const int *done = nullptr);
+void launch_fill(int n, double v, double *dst, const int *done = nullptr);
+// Deterministic dot; result left in *out (device).
+void launch_dot(int n, const double *v, const double *w, double *out);
+
+}  // namespace lap

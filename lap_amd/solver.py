@@ -1,0 +1,209 @@
+"""Host-side mirror of the reference's solver setup, over the C ABI.
+
+Device memory comes from torch (int32 / float64 CUDA tensors); every compute
+call goes through liblapgfx.so.  The setup follows the reference's GPU
+driver (cg_driver2.c:95-351): GS_std gets L+D / D+U with the pivots inside
+the matrices, ichol gets a device copy of A's values factored by
+initialize_ichol, d_r = 1/diag(A) comes from hip_vector_reciprocal.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import lib as L
+from .problems import CSR, augment_lower, augment_upper, diagonal, split
+
+DEV = "cuda"
+
+
+def ptr(t):
+    """Device pointer of a contiguous CUDA tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise L.LapError("expected a contiguous CUDA tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _sync():
+    torch.cuda.synchronize()
+
+
+@dataclass
+class DeviceCSR:
+    n: int
+    ia: torch.Tensor
+    ja: torch.Tensor
+    a: torch.Tensor
+
+    @property
+    def nnz(self) -> int:
+        return int(self.ja.shape[0])
+
+    @classmethod
+    def from_csr(cls, A: CSR, device: str = DEV) -> "DeviceCSR":
+        A.check()
+        return cls(A.n, torch.from_numpy(A.ia).to(device), torch.from_numpy(A.ja).to(device),
+                   torch.from_numpy(np.ascontiguousarray(A.a)).to(device))
+
+
+def to_dev(v: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(DEV)
+
+
+# ----------------------------------------------------------------- kernels
+
+def spmv(A: DeviceCSR, x: torch.Tensor, y: torch.Tensor | None = None, alpha: float = 1.0,
+         beta: float = 0.0) -> torch.Tensor:
+    lib = L.load()
+    if y is None:
+        y = torch.zeros(A.n, dtype=torch.float64, device=x.device)
+    al, be = C.c_double(alpha), C.c_double(beta)
+    _sync()
+    lib.hip_csr_matvec(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a), ptr(x), ptr(y),
+                       C.byref(al), C.byref(be), b"A")
+    L.check()
+    return y
+
+
+def dot(x: torch.Tensor, y: torch.Tensor) -> float:
+    _sync()
+    v = L.load().hip_dot(x.shape[0], ptr(x), ptr(y))
+    L.check()
+    return v
+
+
+def tri_solve(T: DeviceCSR, x: torch.Tensor, lower: bool, diag: torch.Tensor | None = None,
+              out: torch.Tensor | None = None) -> torch.Tensor:
+    lib = L.load()
+    out = torch.empty_like(x) if out is None else out
+    fn = lib.hip_lower_triangular_solve if lower else lib.hip_upper_triangular_solve
+    _sync()
+    fn(T.n, T.nnz, ptr(T.ia), ptr(T.ja), ptr(T.a), ptr(diag), ptr(x), ptr(out))
+    L.check()
+    return out
+
+
+def ichol_factor(A: DeviceCSR) -> torch.Tensor:
+    """IC0 in A's full pattern (lower triangle holds L), as initialize_ichol."""
+    vals = A.a.clone()
+    _sync()
+    L.load().initialize_ichol(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(vals))
+    L.check()
+    return vals
+
+
+# -------------------------------------------------------- preconditioners
+
+@dataclass
+class Preconditioner:
+    """Operands of one preconditioner, kept alive with the pdata pointing at
+    them."""
+    kind: str
+    pdata: L.PData
+    keep: dict = field(default_factory=dict)
+
+
+def make_preconditioner(A: DeviceCSR, host: CSR, kind: str, m: int = 1, k: int = 1,
+                        ) -> Preconditioner:
+    """Mirrors cg_driver2.c's GPU-path setup (:95-144, :219-252, :328-338)."""
+    lib = L.load()
+    n = A.n
+    Ls, d, Us = split(host)
+    if kind == "GS_std":
+        Ls, Us = augment_lower(host), augment_upper(host)
+    Ld, Ud = DeviceCSR.from_csr(Ls), DeviceCSR.from_csr(Us)
+    d_dev = to_dev(d)
+    d_r = torch.empty(n, dtype=torch.float64, device=DEV)
+    _sync()
+    lib.hip_vector_reciprocal(n, ptr(d_dev), ptr(d_r))
+    aux = [torch.zeros(n, dtype=torch.float64, device=DEV) for _ in range(3)]
+    keep = dict(L=Ld, U=Ud, d=d_dev, d_r=d_r, aux=aux, name=kind.encode())
+    ichol_vals = None
+    if kind == "GS_std":
+        lib.initialize_and_analyze_L_and_U_solve(n, Ld.nnz, ptr(Ld.ia), ptr(Ld.ja), ptr(Ld.a),
+                                                 Ud.nnz, ptr(Ud.ia), ptr(Ud.ja), ptr(Ud.a))
+    if kind == "ichol":
+        ichol_vals = A.a.clone()
+        lib.initialize_ichol(n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(ichol_vals))
+        keep["ichol_vals"] = ichol_vals
+    L.check()
+    pd = L.PData(ptr(Ld.ia), ptr(Ld.ja), ptr(Ld.a), Ld.nnz, ptr(Ud.ia), ptr(Ud.ja), ptr(Ud.a),
+                 Ud.nnz, ptr(ichol_vals), ptr(d_dev), ptr(d_r), n, ptr(aux[0]), ptr(aux[1]),
+                 ptr(aux[2]), keep["name"], m, k)
+    return Preconditioner(kind, pd, keep)
+
+
+def prec_apply(A: DeviceCSR, P: Preconditioner, r: torch.Tensor) -> torch.Tensor:
+    w = torch.zeros_like(r)
+    _sync()
+    L.load().prec_function(ptr(A.ia), ptr(A.ja), ptr(A.a), A.nnz, C.byref(P.pdata), ptr(r),
+                           ptr(w))
+    L.check()
+    return w
+
+
+# ------------------------------------------------------------------- PCG
+
+@dataclass
+class Result:
+    x: torch.Tensor
+    iterations: int
+    flag: int
+    history: np.ndarray
+    res0: float
+    true_res: float
+    gpu_ms: float
+
+
+def pcg(A: DeviceCSR, b: torch.Tensor, P: Preconditioner, tol: float = 1e-10,
+        maxit: int = 25000, x0: torch.Tensor | None = None) -> Result:
+    """lap_pcg: the device-resident solve with the reference's recurrence."""
+    lib = L.load()
+    x = torch.zeros(A.n, dtype=torch.float64, device=DEV) if x0 is None else x0.clone()
+    hist = np.zeros(maxit + 2)
+    info = L.SolveInfo()
+    _sync()
+    st = lib.lap_pcg(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a), ptr(x), ptr(b), tol,
+                     C.byref(P.pdata), maxit, hist.ctypes.data_as(C.POINTER(C.c_double)),
+                     C.byref(info))
+    if st != 0:
+        L.check()
+        raise L.LapError(f"lap_pcg returned {st}")
+    return Result(x, info.iterations, info.flag, hist[: info.iterations + 1].copy(), info.res0,
+                  info.true_res, info.gpu_ms)
+
+
+def cg_reference_abi(A: DeviceCSR, b: torch.Tensor, P: Preconditioner, tol: float,
+                     maxit: int):
+    """Calls the exported cg() exactly as the reference's drivers do
+    (cg_driver2.c:392-406); it prints the reference's log lines."""
+    lib = L.load()
+    x = torch.zeros(A.n, dtype=torch.float64, device=DEV)
+    hist = np.zeros(maxit + 2)
+    it, flag = C.c_int(0), C.c_int(0)
+    _sync()
+    lib.cg(A.n, float(A.nnz), ptr(A.ia), ptr(A.ja), ptr(A.a), ptr(x), ptr(b), tol,
+           C.byref(P.pdata), maxit, C.byref(it), C.byref(flag),
+           hist.ctypes.data_as(C.POINTER(C.c_double)))
+    L.check()
+    return x, it.value, flag.value, hist[: it.value + 1].copy()
+
+
+def set_option(name: str, value: int) -> int:
+    return L.load().lap_set_option(name.encode(), int(value))
+
+
+def spmv_timed(A: DeviceCSR, x: torch.Tensor, y: torch.Tensor, reps: int,
+               fused: bool = False) -> float:
+    """Average device ms per SpMV launch (HIP events on the library stream);
+    fused=True times the PCG's q = A p kernel with the p.q epilogue."""
+    _sync()
+    fn = L.load().lap_pcg_spmv_timed if fused else L.load().lap_spmv_timed
+    ms = fn(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a), ptr(x), ptr(y), reps)
+    L.check()
+    return ms

@@ -1,0 +1,42 @@
+"""Small seeded test matrices shared by the CPU and GPU suites."""
+import numpy as np
+
+from lap_amd.problems import from_coo, grid_laplacian
+
+
+def weighted_graph(n, deg, seed, shift=1e-3):
+    """Random weighted graph Laplacian + shift*I (SPD)."""
+    rng = np.random.default_rng(seed)
+    m = n * deg // 2
+    r = rng.integers(0, n, m)
+    c = rng.integers(0, n, m)
+    keep = r != c
+    r, c = np.minimum(r[keep], c[keep]), np.maximum(r[keep], c[keep])
+    key = np.unique(r.astype(np.int64) * n + c)
+    r, c = key // n, key % n
+    w = rng.lognormal(0.0, 1.0, r.shape[0])
+    degw = np.bincount(r, w, n) + np.bincount(c, w, n)
+    rows = np.concatenate([c, np.arange(n)])
+    cols = np.concatenate([r, np.arange(n)])
+    vals = np.concatenate([-w, degw + shift])
+    return from_coo(n, rows, cols, vals, symmetric=True)
+
+
+def star_graph(n, shift=1.0):
+    """Hub row 0 joined to every other node: one row of n nonzeros."""
+    rows = np.concatenate([np.arange(1, n), np.arange(n)])
+    cols = np.concatenate([np.zeros(n - 1, dtype=np.int64), np.arange(n)])
+    deg = np.ones(n)
+    deg[0] = n - 1
+    vals = np.concatenate([-np.ones(n - 1), deg + shift])
+    return from_coo(n, rows, cols, vals, symmetric=True)
+
+
+PROBLEMS = {
+    "grid2d": lambda: grid_laplacian([9, 7], 1e-8),
+    "grid3d": lambda: grid_laplacian([5, 4, 3], 1e-8),
+    "graph": lambda: weighted_graph(300, 8, 7),
+    "grid3d_16": lambda: grid_laplacian([16, 12, 10], 1e-8),
+    "graph_big": lambda: weighted_graph(5000, 12, 11),
+}
+CPU_PROBLEMS = ["grid2d", "grid3d", "graph"]

@@ -1,0 +1,106 @@
+"""Writes the golden fixtures in tests/golden/ from the reference's own CPU
+path, compiled from its sources by oracle/build_ref.sh into oracle/_ref/.
+
+Two independent routes per case:
+  * the reference's cg() in liblapref_cpu.so called on the in-memory CSR
+    (full residual history and solution);
+  * the reference's solver binary lap_cpu_cg run on a Matrix Market file and
+    RHS file written here (iteration count and the summary's 17-digit final
+    residual), which also exercises its MM reader and setup.
+Run in this container:  python tests/golden/make_golden.py [--big]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
+
+import oracle  # noqa: E402
+from lap_amd.problems import grid_laplacian, rhs_normal, rhs_uniform, write_mm, write_rhs  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+SKETCH_ROWS = 16
+
+
+def input_hash(A, b) -> str:
+    h = hashlib.sha256()
+    for arr in (A.ia, A.ja, A.a, b):
+        h.update(np.ascontiguousarray(arr).tobytes())
+    return h.hexdigest()
+
+
+def sketch(x: np.ndarray, seed: int = 777) -> np.ndarray:
+    """Random +-1 projections of x (size-independent solution fingerprint)."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros(SKETCH_ROWS)
+    for k in range(SKETCH_ROWS):
+        s = rng.integers(0, 2, x.shape[0]) * 2.0 - 1.0
+        out[k] = s @ x
+    return out
+
+
+def run_binary(A, b, kind, tol, maxit, m, k):
+    exe = os.path.join(ROOT, "oracle", "_ref", "lap_cpu_cg")
+    with tempfile.TemporaryDirectory() as tmp:
+        mm, rhs = os.path.join(tmp, "A.mtx"), os.path.join(tmp, "b.mtx")
+        write_mm(mm, A)
+        write_rhs(rhs, b)
+        out = subprocess.run([exe, mm, kind, repr(tol), str(maxit), str(m), str(k), rhs],
+                             capture_output=True, text=True, check=True).stdout
+    iters = int(re.search(r"Iters\s*:\s*(\d+)", out).group(1))
+    res = float(re.search(r"Res\. norm\s*:\s*(\S+)", out).group(1))
+    return iters, res
+
+
+def case(name, A, b, kind, tol, maxit=25000, m=1, k=1, full_x=False, binary=True):
+    ref = oracle.Reference()
+    x, it, flag, hist = ref.cg(A, b, kind, tol=tol, maxit=maxit, m=m, k=k)
+    data = dict(iterations=it, flag=flag, history=hist, sketch=sketch(x), xnorm=np.linalg.norm(x),
+                xmean=x.mean(), x_sample=x[:: max(1, A.n // 512)], tol=tol, maxit=maxit, m=m,
+                k=k, kind=kind, input_sha256=input_hash(A, b))
+    if full_x:
+        data["x"] = x
+    if binary:
+        bit, bres = run_binary(A, b, kind, tol, maxit, m, k)
+        assert bit == it, (name, bit, it)
+        data["binary_iterations"] = bit
+        data["binary_res"] = bres
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **data)
+    print(f"{name}: {kind} it={it} flag={flag} res={hist[-1]:.3e}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--big", action="store_true", help="also the 256^3 Jacobi case (slow)")
+    args = ap.parse_args()
+    if not oracle.have_ref():
+        sys.exit("build oracle/_ref first: bash oracle/build_ref.sh")
+    # 2D 5-point 128x128 + 1e-8 I, b ~ U(-1,1) mean-subtracted, tol 1e-8
+    A = grid_laplacian([128, 128], 1e-8)
+    b = rhs_uniform(A.n, 12345)
+    case("c1_line_jacobi", A, b, "line_jacobi", 1e-8, full_x=True)
+    for kind, m, k in (("GS_std", 1, 1), ("ichol", 1, 1), ("it_jacobi", 1, 2),
+                       ("GS_it2", 1, 2), ("none", 1, 1)):
+        case(f"c1_{kind}", A, b, kind, 1e-8, m=m, k=k)
+    # 3D 7-point 32^3 + 1e-8 I, b ~ N(0,1) mean-subtracted, tol 1e-10
+    A = grid_laplacian([32, 32, 32], 1e-8)
+    b = rhs_normal(A.n, 12345)
+    for kind in ("line_jacobi", "GS_std", "ichol"):
+        case(f"g32_{kind}", A, b, kind, 1e-10)
+    if args.big:
+        A = grid_laplacian([256, 256, 256], 1e-8)
+        b = rhs_normal(A.n, 12345)
+        case("c2_line_jacobi", A, b, "line_jacobi", 1e-10, binary=False)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,182 @@
+"""Kernel-level parity of the HIP library (called through the C ABI) with
+the oracle.  Bar: bitwise wherever the reference's arithmetic is a per-row
+sequential recurrence (SpMV rows up to 2048 nonzeros, triangular solves,
+IC0, every preconditioner); the tolerance is stated where the GPU sums in a
+different order (long-row SpMV, dot products)."""
+import numpy as np
+import pytest
+import torch
+
+from cases import PROBLEMS, star_graph
+from lap_amd import solver as S
+from lap_amd.problems import CSR, augment_lower, augment_upper, split
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ["none", "line_jacobi", "it_jacobi", "GS_std", "GS_it", "GS_it2", "ichol"]
+
+
+@pytest.fixture(params=sorted(PROBLEMS))
+def problem(request):
+    print(f"[case] problem={request.param}", flush=True)
+    return PROBLEMS[request.param]()
+
+
+def dev(v):
+    return S.to_dev(v)
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0), (0.75, -1.25)])
+def test_spmv_bitwise(orc, problem, alpha, beta):
+    rng = np.random.default_rng(1)
+    x, y0 = rng.standard_normal(problem.n), rng.standard_normal(problem.n)
+    A = S.DeviceCSR.from_csr(problem)
+    y = S.spmv(A, dev(x), dev(y0), alpha, beta)
+    np.testing.assert_array_equal(host(y), orc.matvec(problem, x, y0, alpha, beta))
+
+
+def test_spmv_kat():
+    A = CSR(5, np.array([0, 3, 4, 7, 9, 10], np.int32),
+            np.array([0, 1, 2, 1, 2, 3, 4, 3, 4, 4], np.int32),
+            np.array([1.0, 2, 3, 7, 4, 1, 2, 3, 1, 9]))
+    y = S.spmv(S.DeviceCSR.from_csr(A), dev(np.array([1.0, 2, 3, 4, 5])), dev(np.ones(5)),
+               -2.0, 1.5)
+    np.testing.assert_array_equal(host(y), [-26.5, -26.5, -50.5, -32.5, -88.5])
+
+
+def test_spmv_long_rows(orc):
+    """A hub row of 6000 nonzeros is summed by a whole workgroup (tree
+    order): within a few ulps of sum |a x|; short rows stay bitwise."""
+    H = star_graph(6000)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(H.n)
+    y = host(S.spmv(S.DeviceCSR.from_csr(H), dev(x)))
+    want = orc.matvec(H, x)
+    absrow = orc.matvec(CSR(H.n, H.ia, H.ja, np.abs(H.a)), np.abs(x))
+    assert abs(y[0] - want[0]) <= 64 * np.finfo(float).eps * absrow[0]
+    np.testing.assert_array_equal(y[1:], want[1:])
+
+
+@pytest.mark.parametrize("rows,xcd", [(64, 1), (512, 1), (256, 0), (1, 1)])
+def test_spmv_blocking_invariant(orc, rows, xcd):
+    """Row-block size and XCD order change scheduling, never the bits."""
+    P = PROBLEMS["graph_big"]()
+    x = np.random.default_rng(3).standard_normal(P.n)
+    old_r, old_x = S.set_option("spmv_rows", rows), S.set_option("spmv_xcd", xcd)
+    try:
+        y = host(S.spmv(S.DeviceCSR.from_csr(P), dev(x)))
+    finally:
+        S.set_option("spmv_rows", old_r)
+        S.set_option("spmv_xcd", old_x)
+    np.testing.assert_array_equal(y, orc.matvec(P, x))
+
+
+def test_dot():
+    rng = np.random.default_rng(4)
+    for n in (1, 63, 1000, 1 << 20):
+        v, w = rng.standard_normal(n), rng.standard_normal(n)
+        got = S.dot(dev(v), dev(w))
+        want = float(np.dot(v, w))
+        bound = 4 * np.log2(n + 1) * np.finfo(float).eps * float(np.abs(v) @ np.abs(w))
+        assert abs(got - want) <= bound + 1e-300
+        assert got == S.dot(dev(v), dev(w))  # deterministic
+
+
+def test_elementwise():
+    from lap_amd import lib as L
+    import ctypes as C
+    lib = L.load()
+    v = np.array([2.0, 0.0, -4.0, 0.25, 9.0])
+    dv, out = dev(v), torch.zeros(5, dtype=torch.float64, device="cuda")
+    lib.hip_vector_reciprocal(5, S.ptr(dv), S.ptr(out))
+    np.testing.assert_array_equal(host(out), [0.5, 0.0, -0.25, 4.0, 1.0 / 9.0])
+    lib.hip_vector_sqrt(5, S.ptr(dv), S.ptr(out))
+    np.testing.assert_array_equal(host(out), [np.sqrt(2.0), 0.0, 0.0, 0.5, 3.0])
+    lib.hip_vec_vec(5, S.ptr(dv), S.ptr(dv), S.ptr(out))
+    np.testing.assert_array_equal(host(out), v * v)
+    lib.hip_vec_set(5, C.c_double(7.5), S.ptr(out))
+    np.testing.assert_array_equal(host(out), np.full(5, 7.5))
+    lib.hip_axpy(5, C.c_double(-2.0), S.ptr(dv), S.ptr(out))
+    np.testing.assert_array_equal(host(out), 7.5 + -2.0 * v)
+    lib.hip_scal(5, C.c_double(0.5), S.ptr(out))
+    np.testing.assert_array_equal(host(out), (7.5 + -2.0 * v) * 0.5)
+    lib.hip_vec_copy(5, S.ptr(dv), S.ptr(out))
+    np.testing.assert_array_equal(host(out), v)
+    lib.hip_vec_zero(5, S.ptr(out))
+    assert not host(out).any()
+    L.check()
+
+
+def test_device_memory_roundtrip():
+    from lap_amd import lib as L
+    import ctypes as C
+    lib = L.load()
+    h = np.arange(1000, dtype=np.float64)
+    back = np.zeros_like(h)
+    d1 = lib.mallocForDevice(None, 1000, 8)
+    d2 = lib.mallocForDevice(None, 1000, 8)
+    assert d1 and d2
+    hp = h.ctypes.data_as(C.c_void_p)
+    bp = back.ctypes.data_as(C.c_void_p)
+    lib.memcpyDevice(d1, hp, 1000, 8, b"H2D")
+    lib.memcpyDevice(d2, d1, 1000, 8, b"D2D")  # the reference routes D2D as D2H
+    lib.memcpyDevice(bp, d2, 1000, 8, b"D2H")
+    lib.freeDevice(d1)
+    lib.freeDevice(d2)
+    L.check()
+    np.testing.assert_array_equal(back, h)
+
+
+def test_trisolve_bitwise(orc, problem):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(problem.n)
+    Ls, d, Us = split(problem)
+    dx, dd = dev(x), dev(d)
+    for T, lower, ref in ((Ls, True, orc.lower_solve), (Us, False, orc.upper_solve)):
+        got = host(S.tri_solve(S.DeviceCSR.from_csr(T), dx, lower, diag=dd))
+        np.testing.assert_array_equal(got, ref(T, x, d))
+    for T, lower, ref in ((augment_lower(problem), True, orc.lower_solve),
+                          (augment_upper(problem), False, orc.upper_solve)):
+        got = host(S.tri_solve(S.DeviceCSR.from_csr(T), dx, lower))
+        np.testing.assert_array_equal(got, ref(T, x))
+
+
+def test_trisolve_kat():
+    U = CSR(5, np.array([0, 2, 2, 4, 5, 5], np.int32), np.array([1, 2, 3, 4, 4], np.int32),
+            np.array([2.0, 3, 1, 2, 1]))
+    got = host(S.tri_solve(S.DeviceCSR.from_csr(U), dev(np.full(5, 8.0)), False,
+                           diag=dev(np.array([1.0, 7, 4, 3, 9]))))
+    np.testing.assert_allclose(got, [2.825397, 1.142857, 0.962963, 2.370370, 0.888889],
+                               atol=5e-7)
+
+
+def test_ichol_bitwise(orc, problem):
+    A = S.DeviceCSR.from_csr(problem)
+    vals = host(S.ichol_factor(A))
+    Lf, _ = orc.ichol_factor(problem)
+    rows = problem.row_ids()
+    low = problem.ja <= rows
+    np.testing.assert_array_equal(vals[low], Lf.a)           # factor in the lower triangle
+    np.testing.assert_array_equal(vals[~low], problem.a[~low])  # upper triangle untouched
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_prec_bitwise(orc, problem, kind):
+    r = np.random.default_rng(6).standard_normal(problem.n)
+    A = S.DeviceCSR.from_csr(problem)
+    P = S.make_preconditioner(A, problem, kind, m=2, k=2)
+    got = host(S.prec_apply(A, P, dev(r)))
+    want = orc.prec_apply(problem, orc.make_prec(problem, kind, m=2, k=2), r)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_spmv_rejects_bad_columns():
+    """Out-of-range column indices are refused before any launch."""
+    from lap_amd import lib as L
+    bad = CSR(3, np.array([0, 1, 2, 3], np.int32), np.array([0, 5, 2], np.int32), np.ones(3))
+    with pytest.raises(L.LapError):
+        S.spmv(S.DeviceCSR.from_csr(bad), dev(np.ones(3)))

@@ -5,38 +5,16 @@ import numpy as np
 import pytest
 import scipy.linalg as sla
 
-from lap_amd.problems import (CSR, augment_lower, augment_upper, from_coo, grid_laplacian,
+from lap_amd.problems import (CSR, augment_lower, augment_upper, grid_laplacian,
                               rhs_normal, rhs_uniform, split)
 
 KINDS = ["none", "line_jacobi", "it_jacobi", "GS_std", "GS_it", "GS_it2", "ichol"]
 
 
-def weighted_graph(n, deg, seed, shift=1e-3):
-    """Random weighted graph Laplacian + shift*I (SPD)."""
-    rng = np.random.default_rng(seed)
-    m = n * deg // 2
-    r = rng.integers(0, n, m)
-    c = rng.integers(0, n, m)
-    keep = r != c
-    r, c = np.minimum(r[keep], c[keep]), np.maximum(r[keep], c[keep])
-    key = np.unique(r.astype(np.int64) * n + c)
-    r, c = key // n, key % n
-    w = rng.lognormal(0.0, 1.0, r.shape[0])
-    degw = np.bincount(r, w, n) + np.bincount(c, w, n)
-    rows = np.concatenate([c, np.arange(n)])
-    cols = np.concatenate([r, np.arange(n)])
-    vals = np.concatenate([-w, degw + shift])
-    return from_coo(n, rows, cols, vals, symmetric=True)
+from cases import CPU_PROBLEMS, PROBLEMS  # noqa: E402
 
 
-PROBLEMS = {
-    "grid2d": lambda: grid_laplacian([9, 7], 1e-8),
-    "grid3d": lambda: grid_laplacian([5, 4, 3], 1e-8),
-    "graph": lambda: weighted_graph(300, 8, 7),
-}
-
-
-@pytest.fixture(params=sorted(PROBLEMS))
+@pytest.fixture(params=CPU_PROBLEMS)
 def problem(request):
     return PROBLEMS[request.param]()
 
