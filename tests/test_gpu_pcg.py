@@ -173,3 +173,46 @@ def test_zero_rhs_converges_immediately():
     res = S.pcg(A, S.to_dev(np.zeros(Ah.n)), P)
     assert (res.iterations, res.flag) == (0, 0)
     assert not res.x.cpu().numpy().any()
+
+
+def _sketch(x, rows=16, seed=777):
+    """Same +-1 projections as tests/golden/make_golden.py::sketch."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros(rows)
+    ones = np.zeros(rows)
+    for k in range(rows):
+        s = rng.integers(0, 2, x.shape[0]) * 2.0 - 1.0
+        out[k] = s @ x
+        ones[k] = s.sum()
+    return out, ones
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "c2_line_jacobi.npz")),
+                    reason="256^3 fixture not generated")
+def test_pcg_256cubed_matches_reference():
+    """Full-size configuration: 256^3 7-point Laplacian + 1e-8 I, Jacobi, tol
+    1e-10, against the reference's own serial cg() (1583 iterations).
+    Size-independent checks: iteration count +-1, early residual history,
+    true residual, and the solution through a 16-row random sign sketch and
+    512 sampled entries (mean removed, see module docstring)."""
+    g = dict(np.load(os.path.join(GOLDEN, "c2_line_jacobi.npz"), allow_pickle=False))
+    Ah = grid_laplacian([256, 256, 256], 1e-8)
+    b = rhs_normal(Ah.n, 12345)
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, "line_jacobi")
+    res = S.pcg(A, S.to_dev(b), P, tol=1e-10, maxit=25000)
+    it_ref = int(g["iterations"])
+    print(f"[diag] 256^3: it {res.iterations} vs {it_ref}, {res.gpu_ms:.1f} ms", flush=True)
+    assert res.flag == 0 and abs(res.iterations - it_ref) <= 1
+    np.testing.assert_allclose(res.history[:10], g["history"][:10], rtol=1e-12)
+    assert res.true_res <= 1e-10 * res.res0 * 1.5
+    x = res.x.cpu().numpy()
+    xs = x[:: max(1, Ah.n // 512)]
+    ds = np.linalg.norm((xs - x.mean()) - (g["x_sample"] - float(g["xmean"])))
+    ds /= np.linalg.norm(g["x_sample"] - float(g["xmean"]))
+    sk, ones = _sketch(x)
+    sk_c = sk - x.mean() * ones
+    ref_c = g["sketch"] - float(g["xmean"]) * ones
+    dk = np.linalg.norm(sk_c - ref_c) / np.linalg.norm(ref_c)
+    print(f"[diag] 256^3: sample rel dev {ds:.2e}, sketch rel dev {dk:.2e}", flush=True)
+    assert ds <= 1e-6 and dk <= 1e-6
