@@ -203,6 +203,7 @@ typedef struct lap_solve_info {
   double res_final;  /* recursive residual norm at exit                     */
   double true_res;   /* ||b - A x|| recomputed at exit                      */
   double gpu_ms;     /* device time of the iteration loop (HIP events)      */
+  int refinements;   /* lap_pcg_mixed: FP64 residual replacements            */
 } lap_solve_info;
 
 /* cg() without printing: the same solve, explicit result.  history (host)
@@ -210,6 +211,21 @@ typedef struct lap_solve_info {
 int lap_pcg(int n, int nnz, const int *ia, const int *ja, const double *a, double *x,
             const double *b, double tol, lap_pdata *prec_data, int maxit, double *history,
             lap_solve_info *info);
+
+/* Mixed-precision PCG for the same system: an FP32 inner PCG (FP32 vectors,
+ * FP32 off-diagonal values of A, FP32 preconditioner) inside FP64 iterative
+ * refinement with reliable updates: whenever the FP32 recursive residual has
+ * dropped by `delta` (0 selects 0.01) since the last refinement, the FP32
+ * correction is added to the FP64 x and the residual is replaced by the FP64
+ * b - A x, without restarting the Krylov process.  FP64 is kept for x, b,
+ * the refinement residual (the caller's FP64 CSR), the diagonal of A, every
+ * row sum and every dot product.  Convergence (||b - A x|| < tol ||r0||) is
+ * decided on FP64 residuals only.  prec_op: line_jacobi, ichol (factor from
+ * initialize_ichol on ichol_vals) or none.  history gets the residual norm
+ * per inner iteration (FP64 at refinements, FP32 recursive otherwise). */
+int lap_pcg_mixed(int n, int nnz, const int *ia, const int *ja, const double *a, double *x,
+                  const double *b, double tol, lap_pdata *prec_data, int maxit, double delta,
+                  double *history, lap_solve_info *info);
 
 /* Kernel-level entry points used by benchmarks and tests (device pointers):
  * y = A x with the fused SpMV, launched `reps` times back to back on the
@@ -222,9 +238,13 @@ double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const do
                           const double *p, double *q, int reps);
 
 /* Library configuration knobs (for tuning sweeps): name/value pairs.
- * "spmv_rows" (rows per SpMV block), "spmv_xcd" (1 = XCD-aware block
- * order), "graph_iters" (iterations per HIP graph launch).  Returns the
- * previous value, or -1 for an unknown name. */
+ * "spmv_rows" (rows per SpMV block), "spmv_walk" (row-block order of the
+ * persistent SpMV grid: 0 cyclic, 1 per-XCD eighths, 2 contiguous chunk per
+ * workgroup), "spmv_variant" (SpMV body, 0-4), "vec_grid" (workgroups of
+ * the vector kernels), "graph_iters" (iterations per HIP graph launch),
+ * "use_graph".  Results do not depend on any of them except vec_grid,
+ * which sets the dot-product summation order.  Returns the previous value,
+ * or -1 for an unknown name. */
 int lap_set_option(const char *name, int value);
 
 #ifdef __cplusplus

@@ -86,6 +86,13 @@ ReduceWs g_ws{};
 DevBufPtr g_ws_buf;
 }  // namespace
 
+int *pinned_flags() {
+  static int *h = nullptr;
+  if (!h && !LAP_HIP(hipHostMalloc((void **)&h, 4 * sizeof(int), hipHostMallocDefault)))
+    h = nullptr;
+  return h;
+}
+
 double *scratch_vec(const char *tag, size_t n) {
   auto &slot = g_scratch[tag];
   if (!slot || slot->bytes < n * sizeof(double)) slot = dev_alloc(n * sizeof(double));
@@ -125,11 +132,17 @@ int lap_set_option(const char *name, int value) {
   Options &o = options();
   int *slot = nullptr;
   if (!strcmp(name, "spmv_rows")) slot = &o.spmv_rows;
-  else if (!strcmp(name, "spmv_xcd")) slot = &o.spmv_xcd;
+  else if (!strcmp(name, "spmv_walk")) slot = &o.spmv_walk;
+  else if (!strcmp(name, "vec_grid")) slot = &o.vec_grid;
   else if (!strcmp(name, "spmv_variant")) slot = &o.spmv_variant;
   else if (!strcmp(name, "graph_iters")) slot = &o.graph_iters;
   else if (!strcmp(name, "use_graph")) slot = &o.use_graph;
   if (!slot) return -1;
+  if ((!strcmp(name, "spmv_walk") && (value < 0 || value > 2)) ||
+      (!strcmp(name, "vec_grid") && (value < 1 || value > kMaxPartialBlocks / 2))) {
+    set_error(LAP_ERR_ARG, "option %s: value %d out of range", name, value);
+    return *slot;
+  }
   if (!strcmp(name, "spmv_variant") && (value < 0 || value > 4)) {
     set_error(LAP_ERR_ARG, "spmv_variant must be in [0, 4]");
     return *slot;
@@ -139,8 +152,7 @@ int lap_set_option(const char *name, int value) {
     return *slot;
   }
   const int prev = *slot;
-  *slot = value;
-  if (!strcmp(name, "spmv_rows")) release_plans();
+  *slot = value;  // SpMV plans are keyed by the row-block size too
   return prev;
 }
 

@@ -68,6 +68,9 @@ std::vector<T> dev_download(const T *dptr, size_t count) {
   return h;
 }
 
+// Two pinned host ints for polling device flags (allocated once).
+int *pinned_flags();
+
 // Cached n-vectors reused across solves (avoids hipMalloc in timed regions).
 double *scratch_vec(const char *tag, size_t n);
 
@@ -125,17 +128,28 @@ __device__ __forceinline__ double sum_partials(const double *partials, int cnt, 
   return r;
 }
 
-// Work distribution of a persistent grid over nblk logical blocks.
-// Workgroups are dispatched round-robin over the 8 XCDs (XCD = id mod 8);
-// with `xcd` set, the workgroups of XCD k walk the k-th contiguous eighth of
-// the logical blocks, so neighbouring rows (which share x entries) are
-// processed under one L2.  Every logical block is visited exactly once.
+// Work distribution of a persistent grid over nblk logical blocks (every
+// logical block is visited exactly once).  Workgroups are dispatched
+// round-robin over the 8 XCDs (XCD = id mod 8), each XCD with its own L2.
+//  mode 0 cyclic:   workgroup w takes blocks w, w+G, w+2G, ...
+//  mode 1 per-XCD:  the workgroups of XCD k walk the k-th contiguous eighth.
+//  mode 2 blocked:  workgroup w takes the contiguous chunk
+//                   [w*C, (w+1)*C), C = ceil(nblk/G).  On a grid-ordered
+//                   matrix the rows that share x entries (neighbouring lines
+//                   and planes) then sit in chunks of workgroups on one XCD.
 struct BlockWalk {
   int begin, end, step;
 };
-__device__ __forceinline__ BlockWalk block_walk(int nblk, int xcd) {
+__device__ __forceinline__ BlockWalk block_walk(int nblk, int mode) {
   const int g = gridDim.x, id = blockIdx.x;
-  if (!xcd || g < 8) return BlockWalk{id, nblk, g};
+  if (mode == 2) {
+    const int chunk = (nblk + g - 1) / g;
+    const long long b0 = (long long)id * chunk;
+    const int begin = b0 < nblk ? (int)b0 : nblk;
+    const int end = begin + chunk < nblk ? begin + chunk : nblk;
+    return BlockWalk{begin, end, 1};
+  }
+  if (mode != 1 || g < 8) return BlockWalk{id, nblk, g};
   const int k = id % 8, lw = id / 8;
   const int nw = g / 8 + (k < g % 8 ? 1 : 0);
   const int lo = (int)(((long long)nblk * k) / 8), hi = (int)(((long long)nblk * (k + 1)) / 8);
@@ -145,7 +159,8 @@ __device__ __forceinline__ BlockWalk block_walk(int nblk, int xcd) {
 // -------------------------------------------------------------- options
 struct Options {
   int spmv_rows = 256;   // rows per SpMV block (<= kBlock * 2)
-  int spmv_xcd = 0;      // XCD-aware block order
+  int spmv_walk = 0;     // row-block walk of the persistent SpMV grid (0,1,2)
+  int vec_grid = 2048;   // workgroups of the streaming vector kernels
   int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;

@@ -1,0 +1,708 @@
+// mixed.hip -- mixed-precision PCG (lap_pcg_mixed): an FP32 inner PCG inside
+// FP64 iterative refinement.
+//
+// Refinement scheme ("reliable updates"): the inner PCG runs on FP32 vectors
+// (r32, p32, q32, a correction x32) with the FP32 copy of A and of the
+// preconditioner.  When its recursive residual has dropped by `delta`
+// relative to the last refinement point (or looks converged, or maxit is
+// reached), the FP32 correction is folded into the FP64 solution
+// (x += x32, x32 = 0), the residual is recomputed in FP64 (r = b - A x with
+// the caller's FP64 matrix) and replaces r32.  The Krylov process is not
+// restarted (p and rho carry over), so the iteration count stays close to
+// the FP64 solver's, whereas a restarted inner solve per refinement step
+// discards the Krylov space every time.  Convergence (||b - A x|| <
+// tol*||r0||) is only ever declared on an FP64 residual.
+//
+// FP64 is used for: the caller's x and b, the refinement residual r = b - A x
+// (FP64 CSR SpMV), the diagonal of A (a shifted graph Laplacian's diagonal
+// degree + 1e-8 does not survive FP32 rounding, which would make the FP32
+// matrix singular), every row sum and every dot product.  FP32 holds: the
+// off-diagonal values of A (own CSR without the diagonal), all inner
+// vectors, 1/diag for Jacobi and the IC0 factor.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lap_core.hpp"
+#include "spmv_core.hpp"
+
+namespace lap {
+
+namespace {
+
+struct MixCtl {
+  double rho;        // r32.w32 of the iteration in flight
+  double rho_prev;   // generic path: previous rho
+  double tol2res0;   // (tol * ||r0||)^2
+  double maxr2;      // squared FP64 residual at the last refinement
+  double delta2;     // refinement trigger (squared)
+  double res0, res;
+  double pend_rho, pend_maxr2;
+  int iter, maxit, done, flag;
+  int pend_iter, pend_valid, last_iter, refinements;
+  int pend_refinements, pad;
+};
+
+constexpr int kQ0 = 0, kQ1 = kMaxPartialBlocks, kQ2 = 2 * kMaxPartialBlocks,
+              kQ3 = 3 * kMaxPartialBlocks;
+
+// Refinement decision of iteration `it`: a pure function of the recursive
+// residual and committed fields, evaluated identically by every kernel.
+__device__ __forceinline__ bool need_refine(double rr, int it, const MixCtl *c) {
+  return !(rr >= c->delta2 * c->maxr2) || rr < c->tol2res0 || it > c->maxit;
+}
+
+// --------------------------------------------------------------- kernels
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int n, const double *__restrict__ s,
+                                                   float *__restrict__ d) {
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n; i += stride)
+    d[i] = (float)s[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_f32(int n, float *__restrict__ d) {
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n; i += stride)
+    d[i] = 0.0f;
+}
+
+// FP64 residual epilogue: r32 = (float)(b - A x); sums of r.r (FP64) and,
+// with Jacobi, r.w and p = w (first direction) for w = d_r r.
+struct Resid64Epi {
+  const double *rhs;
+  float *r32;
+  const float *d_r;  // null: no Jacobi terms
+  float *p_init;     // null: do not set p
+  double rr, rz;
+  __device__ double init(int) const { return 0.0; }
+  __device__ void store(int row, double s) {
+    const double v = -(s - rhs[row]);
+    const float rf = (float)v;
+    r32[row] = rf;
+    rr += v * v;
+    if (d_r) {
+      const float wf = d_r[row] * rf;
+      rz += (double)rf * (double)wf;
+      if (p_init) p_init[row] = wf;
+    }
+  }
+};
+
+// r = b - A x (FP64), initial version: always runs.
+template <int BS, int TILE>
+__global__ __launch_bounds__(BS) void k_mx_init(int nblk, const int *__restrict__ rowblk,
+                                                const int *__restrict__ ia,
+                                                const int *__restrict__ ja,
+                                                const double *__restrict__ a,
+                                                const double *__restrict__ x,
+                                                const double *__restrict__ rhs, float *r32,
+                                                const float *d_r, float *p32, double *part,
+                                                int walk) {
+  __shared__ double prod[TILE];
+  __shared__ double red[BS / kWave];
+  Resid64Epi epi{rhs, r32, d_r, p32, 0.0, 0.0};
+  spmv_walk<BS, TILE, 0>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
+  const double t_rz = block_sum<BS>(epi.rz, red);
+  const double t_rr = block_sum<BS>(epi.rr, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t_rz;
+    part[2 * blockIdx.x + 1] = t_rr;
+  }
+}
+
+// one workgroup: ||r0||, tolerance, initial rho
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_start(const double *part, int cnt, double tol,
+                                                 double delta, MixCtl *ctl, double *hist) {
+  __shared__ double red[BS / kWave];
+  const double rz = sum_partials<BS>(part, cnt, 2, red);
+  const double rr = sum_partials<BS>(part + 1, cnt, 2, red);
+  if (threadIdx.x == 0) {
+    const double res0 = sqrt(rr);
+    hist[0] = res0;
+    ctl->res0 = res0;
+    ctl->res = res0;
+    ctl->tol2res0 = (tol * res0) * (tol * res0);
+    ctl->maxr2 = rr;
+    ctl->delta2 = delta * delta;
+    ctl->rho = rz;
+    if (!isfinite(res0)) {
+      ctl->flag = 2;
+      ctl->done = 1;
+    } else if (res0 == 0.0) {
+      ctl->done = 1;
+    }
+  }
+}
+
+struct Pq32Epi {
+  float *q;
+  const float *p;
+  const double *diag;
+  double acc;
+  __device__ double init(int row) const { return diag[row] * (double)p[row]; }
+  __device__ void store(int row, double s) {
+    const float qf = (float)s;
+    q[row] = qf;
+    acc += (double)p[row] * (double)qf;
+  }
+};
+
+// q32 = A32 p32 (FP32 off-diagonals + FP64 diagonal, FP64 row sums);
+// partials of p.q.  Workgroup 0 commits the pending control fields.
+template <int BS, int TILE>
+__global__ __launch_bounds__(BS) void k_mx_spmv_pq(int nblk, const int *__restrict__ rowblk,
+                                                   const int *__restrict__ ia,
+                                                   const int *__restrict__ ja,
+                                                   const float *__restrict__ a,
+                                                   const double *__restrict__ diag,
+                                                   const float *__restrict__ p,
+                                                   float *__restrict__ q, MixCtl *ctl,
+                                                   double *part, int walk) {
+  if (ctl->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->pend_valid) {
+    ctl->iter = ctl->pend_iter;
+    ctl->rho = ctl->pend_rho;
+    ctl->maxr2 = ctl->pend_maxr2;
+    ctl->refinements = ctl->pend_refinements;
+    ctl->pend_valid = 0;
+  }
+  __shared__ double prod[TILE];
+  __shared__ double red[BS / kWave];
+  Pq32Epi epi{q, p, diag, 0.0};
+  const BlockWalk w = block_walk(nblk, walk);
+  for (int blk = w.begin; blk < w.end; blk += w.step)
+    spmv_block_rows<BS, TILE, Pq32Epi, float, float>(blk, rowblk, ia, ja, a, p, 1.0, prod, red,
+                                                      epi);
+  const double t = block_sum<BS>(epi.acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// alpha = rho/(p.q); x32 += alpha p; r32 -= alpha q (each rounded once from
+// FP64); partials of r.r and, with Jacobi, r.w for w = d_r r (stride 2:
+// [rz, rr]).
+template <int BS, bool JAC>
+__global__ __launch_bounds__(BS) void k_mx_update(int n, float *__restrict__ x32,
+                                                  float *__restrict__ r32,
+                                                  const float *__restrict__ p,
+                                                  const float *__restrict__ q,
+                                                  const float *__restrict__ d_r,
+                                                  const MixCtl *ctl, const double *pq_part,
+                                                  int pq_cnt, double *part) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  const double pq = sum_partials<BS>(pq_part, pq_cnt, 1, red);
+  const double alpha = ctl->rho / pq;
+  double rz = 0.0, rr = 0.0;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+    x32[i] = (float)((double)x32[i] + alpha * (double)p[i]);
+    const float rf = (float)((double)r32[i] - alpha * (double)q[i]);
+    r32[i] = rf;
+    rr += (double)rf * (double)rf;
+    if (JAC) {
+      const float wf = d_r[i] * rf;
+      rz += (double)rf * (double)wf;
+    }
+  }
+  const double t_rz = block_sum<BS>(rz, red);
+  const double t_rr = block_sum<BS>(rr, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t_rz;
+    part[2 * blockIdx.x + 1] = t_rr;
+  }
+}
+
+// refinement, step 1: x += x32, x32 = 0 (only when this iteration refines)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_fold(int n, double *__restrict__ x,
+                                                float *__restrict__ x32, const MixCtl *ctl,
+                                                const double *upd_part, int upd_cnt) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  const double rr = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
+  if (!need_refine(rr, ctl->iter + 1, ctl)) return;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+    x[i] += (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+
+// refinement, step 2: r32 = (float)(b - A x) in FP64; partials [rz, rr]
+template <int BS, int TILE>
+__global__ __launch_bounds__(BS) void k_mx_resid(int nblk, const int *__restrict__ rowblk,
+                                                 const int *__restrict__ ia,
+                                                 const int *__restrict__ ja,
+                                                 const double *__restrict__ a,
+                                                 const double *__restrict__ x,
+                                                 const double *__restrict__ rhs, float *r32,
+                                                 const float *d_r, const MixCtl *ctl,
+                                                 const double *upd_part, int upd_cnt,
+                                                 double *part, int walk) {
+  if (ctl->done) return;
+  __shared__ double prod[TILE];
+  __shared__ double red[BS / kWave];
+  const double rr_rec = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
+  if (!need_refine(rr_rec, ctl->iter + 1, ctl)) return;
+  Resid64Epi epi{rhs, r32, d_r, nullptr, 0.0, 0.0};
+  spmv_walk<BS, TILE, 0>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
+  const double t_rz = block_sum<BS>(epi.rz, red);
+  const double t_rr = block_sum<BS>(epi.rr, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t_rz;
+    part[2 * blockIdx.x + 1] = t_rr;
+  }
+}
+
+// Outcome of an iteration from the update and refinement partials.
+struct MxStep {
+  bool refine;
+  double rz, rr, res;
+  int it, flag;  // flag -1: continue
+};
+
+template <int BS>
+__device__ __forceinline__ MxStep mx_step(const MixCtl *ctl, const double *upd_part, int upd_cnt,
+                                          const double *ref_part, int ref_cnt, double *red) {
+  MxStep s;
+  const double rz_u = sum_partials<BS>(upd_part, upd_cnt, 2, red);
+  const double rr_u = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
+  s.it = ctl->iter + 1;
+  s.refine = need_refine(rr_u, s.it, ctl);
+  s.rz = rz_u;
+  s.rr = rr_u;
+  if (s.refine) {
+    s.rz = sum_partials<BS>(ref_part, ref_cnt, 2, red);
+    s.rr = sum_partials<BS>(ref_part + 1, ref_cnt, 2, red);
+  }
+  s.res = sqrt(s.rr);
+  s.flag = -1;
+  if (s.refine) {  // decisions only on an FP64 residual
+    if (!isfinite(s.res)) s.flag = 2;
+    else if (s.rr < ctl->tol2res0) s.flag = 0;
+    else if (s.it > ctl->maxit) s.flag = 1;
+  }
+  return s;
+}
+
+// Jacobi: convergence bookkeeping, beta, p = beta p + d_r r
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_jac_p(int n, const float *__restrict__ r32,
+                                                 const float *__restrict__ d_r,
+                                                 float *__restrict__ p, MixCtl *ctl,
+                                                 double *hist, const double *upd_part,
+                                                 int upd_cnt, const double *ref_part,
+                                                 int ref_cnt) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  const MxStep s = mx_step<BS>(ctl, upd_part, upd_cnt, ref_part, ref_cnt, red);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hist[s.it] = s.res;
+    ctl->res = s.res;
+    ctl->last_iter = s.it;
+    if (s.flag >= 0) {
+      ctl->flag = s.flag;
+      ctl->refinements = ctl->refinements + 1;
+      ctl->done = 1;
+    } else {
+      ctl->pend_iter = s.it;
+      ctl->pend_rho = s.rz;
+      ctl->pend_maxr2 = s.refine ? s.rr : ctl->maxr2;
+      ctl->pend_refinements = ctl->refinements + (s.refine ? 1 : 0);
+      ctl->pend_valid = 1;
+    }
+  }
+  if (s.flag >= 0) return;
+  const double beta = s.rz / ctl->rho;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+    const float wf = d_r[i] * r32[i];
+    p[i] = (float)((double)p[i] * beta + (double)wf);
+  }
+}
+
+// generic: partials of r.w
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_rw(int n, const float *__restrict__ r,
+                                              const float *__restrict__ w, const MixCtl *ctl,
+                                              double *part) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  double s = 0.0;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride)
+    s += (double)r[i] * (double)w[i];
+  const double t = block_sum<BS>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// generic: rho, p = w (first) or beta p + w
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_pupd(int n, const float *__restrict__ w,
+                                                float *__restrict__ p, MixCtl *ctl,
+                                                const double *part, int cnt) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  const double rho = sum_partials<BS>(part, cnt, 1, red);
+  const bool first = ctl->iter == 0;
+  const double beta = first ? 0.0 : rho / ctl->rho_prev;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rho = rho;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride)
+    p[i] = first ? w[i] : (float)((double)p[i] * beta + (double)w[i]);
+}
+
+// generic, one workgroup: convergence bookkeeping
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_conv(MixCtl *ctl, double *hist,
+                                                const double *upd_part, int upd_cnt,
+                                                const double *ref_part, int ref_cnt) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  const MxStep s = mx_step<BS>(ctl, upd_part, upd_cnt, ref_part, ref_cnt, red);
+  if (threadIdx.x == 0) {
+    hist[s.it] = s.res;
+    ctl->res = s.res;
+    ctl->last_iter = s.it;
+    ctl->iter = s.it;
+    ctl->rho_prev = ctl->rho;
+    if (s.refine) {
+      ctl->maxr2 = s.rr;
+      ctl->refinements = ctl->refinements + 1;
+    }
+    if (s.flag >= 0) {
+      ctl->flag = s.flag;
+      ctl->done = 1;
+    }
+  }
+}
+
+// FP32 triangular solve level (FP64 row sums), pivot = in-row diagonal
+__global__ __launch_bounds__(kBlock) void k_tri32_level(int cnt, const int *__restrict__ rows,
+                                                        const int *__restrict__ ia,
+                                                        const int *__restrict__ ja,
+                                                        const float *__restrict__ a,
+                                                        const float *x, float *y,
+                                                        const int *__restrict__ done) {
+  if (done && *done) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  const int row = rows[t];
+  double acc = (double)x[row];
+  double piv = 0.0;
+  const int e1 = ia[row + 1];
+  for (int e = ia[row]; e < e1; ++e) {
+    const int c = ja[e];
+    if (c == row) {
+      piv = (double)a[e];
+      continue;
+    }
+    acc -= (double)a[e] * (double)y[c];
+  }
+  y[row] = (float)(acc / piv);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_f32(int n, const float *__restrict__ s,
+                                                     float *__restrict__ d,
+                                                     const int *__restrict__ done) {
+  if (done && *done) return;
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n; i += stride)
+    d[i] = s[i];
+}
+
+// ---------------------------------------------------------------- plans
+
+struct MixPlan {
+  int n = 0, nnz_off = 0;
+  DevBufPtr ia_o, ja_o, a32_o, diag;
+  const SpmvPlan *off = nullptr;
+  std::map<const double *, DevBufPtr> dr32;  // by d_r pointer
+  std::map<const IcPlan *, std::pair<DevBufPtr, DevBufPtr>> ic32;
+};
+using MixKey = std::tuple<const int *, const int *, const double *, int, int>;
+std::map<MixKey, std::unique_ptr<MixPlan>> g_mix;
+
+MixPlan *mix_plan(int n, int nnz, const int *ia, const int *ja, const double *a) {
+  MixKey key{ia, ja, a, n, nnz};
+  auto it = g_mix.find(key);
+  if (it != g_mix.end()) return it->second.get();
+  std::vector<int> h_ia = dev_download(ia, (size_t)n + 1);
+  const int total = h_ia[n];
+  std::vector<int> h_ja = dev_download(ja, (size_t)total);
+  std::vector<double> h_a = dev_download(a, (size_t)total);
+  std::vector<int> io((size_t)n + 1, 0), jo;
+  std::vector<float> ao;
+  std::vector<double> dg((size_t)n, 0.0);
+  jo.reserve((size_t)total);
+  ao.reserve((size_t)total);
+  for (int i = 0; i < n; ++i) {
+    for (int e = h_ia[i]; e < h_ia[i + 1]; ++e) {
+      if (h_ja[e] == i) {
+        dg[i] += h_a[e];
+      } else {
+        jo.push_back(h_ja[e]);
+        ao.push_back((float)h_a[e]);
+      }
+    }
+    io[i + 1] = (int)jo.size();
+  }
+  auto P = std::make_unique<MixPlan>();
+  P->n = n;
+  P->nnz_off = (int)jo.size();
+  P->ia_o = dev_upload(io);
+  P->ja_o = dev_upload(jo);
+  P->a32_o = dev_upload(ao);
+  P->diag = dev_upload(dg);
+  P->off = spmv_plan(n, P->nnz_off, P->ia_o->as<int>(), P->ja_o->as<int>());
+  if (!P->off) return nullptr;
+  MixPlan *raw = P.get();
+  g_mix[key] = std::move(P);
+  return raw;
+}
+
+float *to_f32_cached(std::map<const double *, DevBufPtr> &cache, const double *src, int n) {
+  auto &slot = cache[src];
+  if (!slot) {
+    slot = dev_alloc(sizeof(float) * (size_t)n);
+    k_to_f32<<<grid_for((size_t)n, 4, kBlock, 2048), kBlock, 0, stream()>>>(n, src,
+                                                                            slot->as<float>());
+  }
+  return slot->as<float>();
+}
+
+}  // namespace
+
+void release_mix_plans() { g_mix.clear(); }
+
+struct MixOut {
+  int iterations = 0, flag = 2, refinements = 0;
+  double res0 = 0, res = 0, gpu_ms = 0;
+};
+
+static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double *a, double *x,
+                     const double *b, double tol, lap_pdata *pd, int maxit, double delta,
+                     double *h_hist, MixOut &out) {
+  if (n <= 0 || nnz < 0 || maxit < 0 || !(delta > 0.0 && delta < 1.0)) {
+    set_error(LAP_ERR_ARG, "pcg_mixed: bad arguments n=%d maxit=%d delta=%g", n, maxit, delta);
+    return LAP_ERR_ARG;
+  }
+  for (const void *ptr : {(const void *)ia, (const void *)ja, (const void *)a, (const void *)x,
+                          (const void *)b})
+    if (!is_device_ptr(ptr)) {
+      set_error(LAP_ERR_ARG, "pcg_mixed: argument %p is not device memory", ptr);
+      return LAP_ERR_ARG;
+    }
+  const char *op = pd && pd->prec_op ? pd->prec_op : "";
+  const bool jac = !strcmp(op, "line_jacobi");
+  const bool ic = !strcmp(op, "ichol");
+  const bool none = !strcmp(op, "none");
+  if (!jac && !ic && !none) {
+    set_error(LAP_ERR_PREC, "pcg_mixed: preconditioner '%s' not supported (line_jacobi, ichol, "
+              "none)", op);
+    return LAP_ERR_PREC;
+  }
+  const SpmvPlan *A64 = spmv_plan(n, nnz, ia, ja);
+  if (!A64) return lap_last_status();
+  MixPlan *M = mix_plan(n, nnz, ia, ja, a);
+  if (!M) return lap_last_status();
+  const float *d_r32 = nullptr;
+  if (jac) {
+    if (!is_device_ptr(pd->d_r)) {
+      set_error(LAP_ERR_ARG, "pcg_mixed: prec_data->d_r is not device memory");
+      return LAP_ERR_ARG;
+    }
+    d_r32 = to_f32_cached(M->dr32, pd->d_r, n);
+  }
+  const IcPlan *icp = nullptr;
+  float *l32 = nullptr, *u32 = nullptr;
+  if (ic) {
+    icp = ic_plan_find(pd->ichol_vals);
+    if (!icp) {
+      set_error(LAP_ERR_PREC, "pcg_mixed: no IC0 factor for ichol_vals (call initialize_ichol)");
+      return LAP_ERR_PREC;
+    }
+    auto &slot = M->ic32[icp];
+    if (!slot.first) {
+      slot.first = dev_alloc(sizeof(float) * (size_t)std::max(icp->nnzL, 1));
+      slot.second = dev_alloc(sizeof(float) * (size_t)std::max(icp->nnzL, 1));
+      const unsigned g = grid_for((size_t)icp->nnzL, 4, kBlock, 2048);
+      k_to_f32<<<g, kBlock, 0, stream()>>>(icp->nnzL, icp->lval->as<double>(),
+                                           slot.first->as<float>());
+      k_to_f32<<<g, kBlock, 0, stream()>>>(icp->nnzL, icp->uval->as<double>(),
+                                           slot.second->as<float>());
+    }
+    l32 = slot.first->as<float>();
+    u32 = slot.second->as<float>();
+  }
+
+  auto fvec = [&](const char *tag) {
+    return reinterpret_cast<float *>(scratch_vec(tag, (size_t)n / 2 + 1));
+  };
+  float *r32 = fvec("mx_r"), *p32 = fvec("mx_p"), *q32 = fvec("mx_q"), *x32 = fvec("mx_x"),
+        *w32 = fvec("mx_w"), *aux32 = fvec("mx_aux");
+  MixCtl *ctl = reinterpret_cast<MixCtl *>(scratch_vec("mx_ctl", sizeof(MixCtl) / 8 + 1));
+  double *hist = scratch_vec("mx_hist", (size_t)maxit + 2);
+  const int *done = &ctl->done;
+  ReduceWs &ws = reduce_ws();
+  double *Q0 = ws.partials + kQ0, *Q1 = ws.partials + kQ1, *Q2 = ws.partials + kQ2,
+         *Q3 = ws.partials + kQ3;
+  const unsigned gA = spmv_grid(*A64), gO = spmv_grid(*M->off);
+  const unsigned gV = grid_for((size_t)n, 4, kBlock, (unsigned)options().vec_grid);
+  const int walk = options().spmv_walk;
+
+  MixCtl h0{};
+  h0.maxit = maxit;
+  LAP_HIP(hipMemcpyAsync(ctl, &h0, sizeof h0, hipMemcpyHostToDevice, stream()));
+  hipEvent_t ev0, ev1;
+  LAP_HIP(hipEventCreate(&ev0));
+  LAP_HIP(hipEventCreate(&ev1));
+  LAP_HIP(hipEventRecord(ev0, stream()));
+  k_zero_f32<<<gV, kBlock, 0, stream()>>>(n, x32);
+  k_mx_init<kBlock, kSpmvTile><<<gA, kBlock, 0, stream()>>>(
+      A64->nblocks, A64->rowblk->as<int>(), ia, ja, a, x, b, r32, d_r32, jac ? p32 : nullptr,
+      Q0, walk);
+  k_mx_start<kBlock><<<1, kBlock, 0, stream()>>>(Q0, (int)gA, tol, delta, ctl, hist);
+
+  const int *rb_o = M->off->rowblk->as<int>(), *rb_a = A64->rowblk->as<int>();
+  const int *io = M->ia_o->as<int>(), *jo = M->ja_o->as<int>();
+  const float *ao = M->a32_o->as<float>();
+  const double *dg = M->diag->as<double>();
+  auto tri32 = [&](const TriPlan &T, const int *tia, const int *tja, const float *ta,
+                   const float *in, float *outv) {
+    const int *order = T.order->as<int>();
+    for (int l = 0; l < T.nlevels; ++l) {
+      const int cnt = T.level_ptr[l + 1] - T.level_ptr[l];
+      k_tri32_level<<<(cnt + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(
+          cnt, order + T.level_ptr[l], tia, tja, ta, in, outv, done);
+    }
+  };
+  auto iteration = [&]() {
+    if (jac) {
+      k_mx_spmv_pq<kBlock, kSpmvTile><<<gO, kBlock, 0, stream()>>>(M->off->nblocks, rb_o, io,
+                                                                   jo, ao, dg, p32, q32, ctl,
+                                                                   Q1, walk);
+      k_mx_update<kBlock, true><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, d_r32, ctl,
+                                                             Q1, (int)gO, Q2);
+      k_mx_fold<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
+      k_mx_resid<kBlock, kSpmvTile><<<gA, kBlock, 0, stream()>>>(
+          A64->nblocks, rb_a, ia, ja, a, x, b, r32, d_r32, ctl, Q2, (int)gV, Q3, walk);
+      k_mx_jac_p<kBlock><<<gV, kBlock, 0, stream()>>>(n, r32, d_r32, p32, ctl, hist, Q2,
+                                                      (int)gV, Q3, (int)gA);
+    } else {
+      if (ic) {
+        tri32(*icp->lplan, icp->lia->as<int>(), icp->lja->as<int>(), l32, r32, aux32);
+        tri32(*icp->uplan, icp->uia->as<int>(), icp->uja->as<int>(), u32, aux32, w32);
+      } else {
+        k_copy_f32<<<gV, kBlock, 0, stream()>>>(n, r32, w32, done);
+      }
+      k_mx_rw<kBlock><<<gV, kBlock, 0, stream()>>>(n, r32, w32, ctl, Q0);
+      k_mx_pupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, w32, p32, ctl, Q0, (int)gV);
+      k_mx_spmv_pq<kBlock, kSpmvTile><<<gO, kBlock, 0, stream()>>>(M->off->nblocks, rb_o, io,
+                                                                   jo, ao, dg, p32, q32, ctl,
+                                                                   Q1, walk);
+      k_mx_update<kBlock, false><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, nullptr,
+                                                              ctl, Q1, (int)gO, Q2);
+      k_mx_fold<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
+      k_mx_resid<kBlock, kSpmvTile><<<gA, kBlock, 0, stream()>>>(
+          A64->nblocks, rb_a, ia, ja, a, x, b, r32, nullptr, ctl, Q2, (int)gV, Q3, walk);
+      k_mx_conv<kBlock><<<1, kBlock, 0, stream()>>>(ctl, hist, Q2, (int)gV, Q3, (int)gA);
+    }
+  };
+  int per_iter = jac ? 5 : 8;
+  if (ic) per_iter += icp->lplan->nlevels + icp->uplan->nlevels;
+  int G = std::max(1, std::min(options().graph_iters, 4096 / per_iter));
+  G = std::min(G, maxit + 2);
+
+  int status = LAP_OK;
+  hipGraphExec_t gexec = nullptr;
+  if (options().use_graph) {
+    hipGraph_t graph = nullptr;
+    bool ok = LAP_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    if (ok) {
+      for (int k = 0; k < G; ++k) iteration();
+      ok = LAP_HIP(hipStreamEndCapture(stream(), &graph));
+    }
+    if (ok) ok = LAP_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+      gexec = nullptr;
+      status = LAP_ERR_HIP;
+    }
+  }
+  int *flags = pinned_flags();
+  if (!flags) status = LAP_ERR_HIP;
+  if (status == LAP_OK) {
+    hipEvent_t pe[2];
+    LAP_HIP(hipEventCreateWithFlags(&pe[0], hipEventDisableTiming));
+    LAP_HIP(hipEventCreateWithFlags(&pe[1], hipEventDisableTiming));
+    const long max_batches = (long)(maxit + 1) / G + 3;
+    bool finished = false;
+    for (long k = 0; k < max_batches + 1 && !finished; ++k) {
+      if (k < max_batches) {
+        if (gexec) {
+          if (!LAP_HIP(hipGraphLaunch(gexec, stream()))) break;
+        } else {
+          for (int s = 0; s < G; ++s) iteration();
+        }
+        LAP_HIP(hipMemcpyAsync(flags + (k & 1), done, sizeof(int), hipMemcpyDeviceToHost,
+                               stream()));
+        LAP_HIP(hipEventRecord(pe[k & 1], stream()));
+      }
+      if (k > 0) {
+        LAP_HIP(hipEventSynchronize(pe[(k - 1) & 1]));
+        if (flags[(k - 1) & 1]) finished = true;
+      }
+    }
+    if (!finished) {
+      set_error(LAP_ERR_NUMERIC, "pcg_mixed: convergence flag never set");
+      status = LAP_ERR_NUMERIC;
+    }
+    (void)hipEventDestroy(pe[0]);
+    (void)hipEventDestroy(pe[1]);
+  }
+  LAP_HIP(hipEventRecord(ev1, stream()));
+  sync();
+  float ms = 0.f;
+  LAP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  MixCtl h{};
+  LAP_HIP(hipMemcpy(&h, ctl, sizeof h, hipMemcpyDeviceToHost));
+  out.iterations = h.last_iter;
+  out.flag = h.flag;
+  out.refinements = h.refinements;
+  out.res0 = h.res0;
+  out.res = h.res;
+  out.gpu_ms = ms;
+  if (h_hist)
+    LAP_HIP(hipMemcpy(h_hist, hist, sizeof(double) * (size_t)(h.last_iter + 1),
+                      hipMemcpyDeviceToHost));
+  if (status == LAP_OK && lap_last_status() == LAP_ERR_HIP) status = LAP_ERR_HIP;
+  return status;
+}
+
+}  // namespace lap
+
+using namespace lap;
+
+extern "C" int lap_pcg_mixed(int n, int nnz, const int *ia, const int *ja, const double *a,
+                             double *x, const double *b, double tol, lap_pdata *prec_data,
+                             int maxit, double delta, double *history, lap_solve_info *info) {
+  MixOut out;
+  if (delta == 0.0) delta = 0.01;
+  const int st = run_mixed(n, nnz, ia, ja, a, x, b, tol, prec_data, maxit, delta, history, out);
+  if (info) {
+    info->iterations = out.iterations;
+    info->flag = st == LAP_OK ? out.flag : 2;
+    info->res0 = out.res0;
+    info->res_final = out.res;
+    info->true_res = out.res;  // the final residual is an FP64 one
+    info->gpu_ms = out.gpu_ms;
+    info->refinements = out.refinements;
+  }
+  return st;
+}

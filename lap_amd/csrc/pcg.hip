@@ -56,12 +56,13 @@ static PrecKind prec_kind(const char *s) {
 
 namespace {
 
-constexpr unsigned kVecGridCap = 2048;
 // partial-sum regions of the reduction workspace (kMaxPartialBlocks each)
 constexpr int kR0 = 0, kR1 = kMaxPartialBlocks, kR2 = 2 * kMaxPartialBlocks,
               kR3 = 3 * kMaxPartialBlocks;
 
-inline unsigned vgrid(int n) { return grid_for((size_t)(n > 0 ? n : 1), 4, kBlock, kVecGridCap); }
+inline unsigned vgrid(int n) {
+  return grid_for((size_t)(n > 0 ? n : 1), 4, kBlock, (unsigned)options().vec_grid);
+}
 
 // ---------------------------------------------------------------- kernels
 //
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(BS) void k_conv(PcgCtl *ctl, double *hist, const do
 static void launch_init(unsigned g, const SpmvPlan &A, const int *ia, const int *ja,
                         const double *a, const double *x, const double *b, double *r,
                         double *part) {
-  const int xcd = options().spmv_xcd;
+  const int xcd = options().spmv_walk;
   const int *rb = A.rowblk->as<int>();
 #define LAP_CASE(V)                                                                       \
   case V:                                                                                 \
@@ -358,7 +359,7 @@ static void launch_init(unsigned g, const SpmvPlan &A, const int *ia, const int 
 static void launch_spmv_pq(unsigned g, const SpmvPlan &A, const int *ia, const int *ja,
                            const double *a, const double *p, double *q, PcgCtl *ctl,
                            double *part) {
-  const int xcd = options().spmv_xcd;
+  const int xcd = options().spmv_walk;
   const int *rb = A.rowblk->as<int>();
 #define LAP_CASE(V)                                                                          \
   case V:                                                                                    \
@@ -569,25 +570,6 @@ static void prec_launch(const PrecOps &o, const double *r, double *w, const int 
 
 // ------------------------------------------------------------- the solve
 
-namespace {
-
-struct PinnedFlags {
-  int *h = nullptr;
-  PinnedFlags() {
-    if (!LAP_HIP(hipHostMalloc((void **)&h, 4 * sizeof(int), hipHostMallocDefault))) h = nullptr;
-  }
-  ~PinnedFlags() {
-    if (h) (void)hipHostFree(h);
-  }
-};
-
-PinnedFlags &pinned() {
-  static PinnedFlags f;
-  return f;
-}
-
-}  // namespace
-
 struct SolveOut {
   int iterations = 0, flag = 2;
   double res0 = 0, res = 0, gpu_ms = 0;
@@ -661,7 +643,8 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   G = std::min(G, maxit + 2);
 
   int status = LAP_OK;
-  int *flags = pinned().h;
+  int *flags = pinned_flags();
+  if (!flags) status = LAP_ERR_HIP;
   hipGraphExec_t gexec = nullptr;
   if (options().use_graph) {
     hipGraph_t graph = nullptr;

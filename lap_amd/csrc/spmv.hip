@@ -118,7 +118,7 @@ void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *
                  const double *x, double *y, double alpha, double beta, const int *done) {
   if (p.nblocks == 0) return;
   const unsigned g = spmv_grid(p);
-  const int xcd = options().spmv_xcd;
+  const int xcd = options().spmv_walk;
   const int *rb = p.rowblk->as<int>();
 #define LAP_SPMV_CASE(V)                                                                   \
   case V:                                                                                  \
@@ -138,7 +138,9 @@ void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *
 
 void release_tri_plans();
 void release_ic_plans();
+void release_mix_plans();
 void release_plans() {
+  release_mix_plans();
   g_plans.clear();
   release_tri_plans();
   release_ic_plans();

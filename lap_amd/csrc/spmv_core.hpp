@@ -34,13 +34,14 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 
 // Long row (more than TILE nonzeros, a row block of its own): strided
 // partial sums and a fixed-order tree over the workgroup.
-template <int BS, class Epi>
+template <int BS, class VT, class XT, class Epi>
 __device__ __forceinline__ void spmv_long_row(int r0, int e0, int e1, const int *__restrict__ ja,
-                                              const double *__restrict__ a,
-                                              const double *__restrict__ x, double alpha,
+                                              const VT *__restrict__ a,
+                                              const XT *__restrict__ x, double alpha,
                                               double *red, Epi &epi) {
   double s = 0.0;
-  for (int e = e0 + (int)threadIdx.x; e < e1; e += BS) s += (alpha * a[e]) * x[ja[e]];
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += BS)
+    s += (alpha * (double)a[e]) * (double)x[ja[e]];
   const double t = block_sum<BS>(s, red);
   if (threadIdx.x == 0) epi.store(r0, epi.init(r0) + t);
 }
@@ -58,14 +59,15 @@ __device__ __forceinline__ void spmv_row_sums(int r0, int r1, int e0, const int 
   }
 }
 
-// Variant 0 body for row block b.  Block-uniform control flow.
+// Variant 0 body for row block b.  Block-uniform control flow.  Values
+// (VT) and vectors (XT) may be FP32; products and row sums are FP64.
 //  prod : __shared__ double[TILE], red : __shared__ double[BS/64].
-template <int BS, int TILE, class Epi>
+template <int BS, int TILE, class Epi, class VT = double, class XT = double>
 __device__ __forceinline__ void spmv_block_rows(int b, const int *__restrict__ rowblk,
                                                 const int *__restrict__ ia,
                                                 const int *__restrict__ ja,
-                                                const double *__restrict__ a,
-                                                const double *__restrict__ x, double alpha,
+                                                const VT *__restrict__ a,
+                                                const XT *__restrict__ x, double alpha,
                                                 double *prod, double *red, Epi &epi) {
   const int r0 = rowblk[b], r1 = rowblk[b + 1];
   const int e0 = ia[r0], e1 = ia[r1];
@@ -77,7 +79,7 @@ __device__ __forceinline__ void spmv_block_rows(int b, const int *__restrict__ r
 #pragma unroll 4
   for (int k = threadIdx.x; k < cnt; k += BS) {
     const int e = e0 + k;
-    prod[k] = (alpha * a[e]) * x[ja[e]];
+    prod[k] = (alpha * (double)a[e]) * (double)x[ja[e]];
   }
   __syncthreads();
   spmv_row_sums<BS>(r0, r1, e0, ia, prod, epi);

@@ -31,7 +31,8 @@ class PData(C.Structure):
 
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("flag", C.c_int), ("res0", C.c_double),
-                ("res_final", C.c_double), ("true_res", C.c_double), ("gpu_ms", C.c_double)]
+                ("res_final", C.c_double), ("true_res", C.c_double), ("gpu_ms", C.c_double),
+                ("refinements", C.c_int)]
 
 
 # name -> (restype, argtypes)
@@ -67,6 +68,9 @@ _SIGS = {
     "lap_release_all": (None, []),
     "lap_pcg": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_double, C.POINTER(PData),
                           C.c_int, C.POINTER(C.c_double), C.POINTER(SolveInfo)]),
+    "lap_pcg_mixed": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_double,
+                                C.POINTER(PData), C.c_int, C.c_double, C.POINTER(C.c_double),
+                                C.POINTER(SolveInfo)]),
     "lap_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
     "lap_pcg_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
     "lap_set_option": (C.c_int, [C.c_char_p, C.c_int]),

@@ -207,3 +207,27 @@ def spmv_timed(A: DeviceCSR, x: torch.Tensor, y: torch.Tensor, reps: int,
     ms = fn(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a), ptr(x), ptr(y), reps)
     L.check()
     return ms
+
+
+@dataclass
+class MixedResult(Result):
+    refinements: int = 0
+
+
+def pcg_mixed(A: DeviceCSR, b: torch.Tensor, P: Preconditioner, tol: float = 1e-10,
+              maxit: int = 25000, delta: float = 0.01,
+              x0: torch.Tensor | None = None) -> MixedResult:
+    """lap_pcg_mixed: FP32 inner PCG in FP64 reliable-update refinement."""
+    lib = L.load()
+    x = torch.zeros(A.n, dtype=torch.float64, device=DEV) if x0 is None else x0.clone()
+    hist = np.zeros(maxit + 2)
+    info = L.SolveInfo()
+    _sync()
+    st = lib.lap_pcg_mixed(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a), ptr(x), ptr(b), tol,
+                           C.byref(P.pdata), maxit, delta,
+                           hist.ctypes.data_as(C.POINTER(C.c_double)), C.byref(info))
+    if st != 0:
+        L.check()
+        raise L.LapError(f"lap_pcg_mixed returned {st}")
+    return MixedResult(x, info.iterations, info.flag, hist[: info.iterations + 1].copy(),
+                       info.res0, info.true_res, info.gpu_ms, info.refinements)

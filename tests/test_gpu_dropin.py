@@ -1,0 +1,49 @@
+"""Drop-in check: the reference's own host code (cg_driver2.c, cg.c, prec.c,
+GS.c, it_jacobi.c, line_jacobi.c, blas.c, io_utils.c compiled with -DHIP=1 by
+oracle/build_ref.sh) linked against liblapgfx.so instead of hip_blas.o +
+rocSPARSE/rocBLAS, run on a Matrix Market input, agrees with the reference
+CPU binary on the same files: same exit reason, iteration count within +-1."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from lap_amd.problems import grid_laplacian, rhs_uniform, write_mm, write_rhs
+
+pytestmark = pytest.mark.gpu
+
+REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref")
+DROPIN = os.path.join(REF, "lap_dropin_cg")
+CPU = os.path.join(REF, "lap_cpu_cg")
+
+
+def run(exe, args):
+    out = subprocess.run([exe, *args], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    text = out.stdout
+    iters = int(re.search(r"Iters\s*:\s*(\d+)", text).group(1))
+    res = float(re.search(r"Res\. norm\s*:\s*(\S+)", text).group(1))
+    reason = re.search(r"Reason for exiting : (.*)", text).group(1).strip()
+    return iters, res, reason, text
+
+
+@pytest.mark.skipif(not (os.path.exists(DROPIN) and os.path.exists(CPU)),
+                    reason="drop-in binary not built (needs the reference sources)")
+@pytest.mark.parametrize("kind", ["line_jacobi", "GS_std", "ichol", "it_jacobi"])
+def test_reference_driver_on_gfx950_backend(tmp_path, kind):
+    print(f"[case] dropin kind={kind}", flush=True)
+    A = grid_laplacian([128, 128], 1e-8)
+    b = rhs_uniform(A.n, 12345)
+    mm, rhs = str(tmp_path / "A.mtx"), str(tmp_path / "b.mtx")
+    write_mm(mm, A)
+    write_rhs(rhs, b)
+    k = "2" if kind == "it_jacobi" else "1"
+    args = [mm, kind, "1e-8", "25000", "1", k, rhs]
+    gi, gres, greason, gtext = run(DROPIN, args)
+    ci, cres, creason, _ = run(CPU, args)
+    print(f"[diag] dropin {kind}: gpu it {gi} res {gres:.6e} ({greason}); "
+          f"cpu it {ci} res {cres:.6e} ({creason})", flush=True)
+    assert greason == creason == "CG converged"
+    assert abs(gi - ci) <= 1
+    assert "TRUE Norm of r" in gtext

@@ -61,17 +61,20 @@ def test_spmv_long_rows(orc):
     np.testing.assert_array_equal(y[1:], want[1:])
 
 
-@pytest.mark.parametrize("rows,xcd", [(64, 1), (512, 1), (256, 0), (1, 1)])
-def test_spmv_blocking_invariant(orc, rows, xcd):
-    """Row-block size and XCD order change scheduling, never the bits."""
+@pytest.mark.parametrize("rows,walk,variant", [(64, 1, 0), (512, 2, 0), (256, 0, 1), (1, 1, 2),
+                                               (256, 2, 3), (128, 0, 4)])
+def test_spmv_blocking_invariant(orc, rows, walk, variant):
+    """Row-block size, walk order and body variant change scheduling, never
+    the bits."""
     P = PROBLEMS["graph_big"]()
     x = np.random.default_rng(3).standard_normal(P.n)
-    old_r, old_x = S.set_option("spmv_rows", rows), S.set_option("spmv_xcd", xcd)
+    old = [S.set_option("spmv_rows", rows), S.set_option("spmv_walk", walk),
+           S.set_option("spmv_variant", variant)]
     try:
         y = host(S.spmv(S.DeviceCSR.from_csr(P), dev(x)))
     finally:
-        S.set_option("spmv_rows", old_r)
-        S.set_option("spmv_xcd", old_x)
+        for name, v in zip(("spmv_rows", "spmv_walk", "spmv_variant"), old):
+            S.set_option(name, v)
     np.testing.assert_array_equal(y, orc.matvec(P, x))
 
 

@@ -93,7 +93,8 @@ def test_pcg_equal_iterations_vs_oracle(orc, kind, pname):
     res = S.pcg(A, S.to_dev(b), P, tol=1e-30, maxit=maxit)
     xo, ito, flo, ho = orc.cg(Ah, b, kind, tol=1e-30, maxit=maxit, m=1, k=k)
     assert (res.iterations, res.flag) == (ito, flo) == (maxit + 1, 1)
-    np.testing.assert_allclose(res.history, ho, rtol=1e-10)
+    # residuals below ~1e-11 ||r0|| are rounding noise: absolute floor there
+    np.testing.assert_allclose(res.history, ho, rtol=1e-10, atol=1e-12 * ho[0])
     assert rel(res.x.cpu().numpy(), xo) <= 1e-10
 
 
