@@ -141,6 +141,8 @@ using namespace lap;
 extern "C" {
 
 double hip_dot(const int n, const double *v, const double *w) {
+  if (n <= 0) return 0.0;
+  if (!check_device("hip_dot", {{v, "v"}, {w, "w"}})) return 0.0;
   ReduceWs &ws = reduce_ws();
   launch_dot(n, v, w, ws.scalars + 0);
   double h = 0.0;
@@ -150,42 +152,50 @@ double hip_dot(const int n, const double *v, const double *w) {
 }
 
 void hip_scal(const int n, const double alpha, double *v) {
+  if (n <= 0 || !check_device("hip_scal", {{v, "v"}})) return;
   launch_scal(n, alpha, v);
   sync();
 }
 
 void hip_axpy(const int n, const double alpha, const double *x, double *y) {
+  if (n <= 0 || !check_device("hip_axpy", {{x, "x"}, {y, "y"}})) return;
   launch_axpy(n, alpha, x, y);
   sync();
 }
 
 void hip_vec_vec(const int n, const double *x, const double *y, double *res) {
+  if (n <= 0 || !check_device("hip_vec_vec", {{x, "x"}, {y, "y"}, {res, "res"}})) return;
   launch_vec_vec(n, x, y, res);
   sync();
 }
 
 void hip_vector_reciprocal(const int n, const double *v, double *res) {
-  if (n > 0) k_reciprocal<<<sgrid(n), kBlock, 0, stream()>>>(n, v, res);
+  if (n <= 0 || !check_device("hip_vector_reciprocal", {{v, "v"}, {res, "res"}})) return;
+  k_reciprocal<<<sgrid(n), kBlock, 0, stream()>>>(n, v, res);
   sync();
 }
 
 void hip_vector_sqrt(const int n, const double *v, double *res) {
-  if (n > 0) k_sqrt<<<sgrid(n), kBlock, 0, stream()>>>(n, v, res);
+  if (n <= 0 || !check_device("hip_vector_sqrt", {{v, "v"}, {res, "res"}})) return;
+  k_sqrt<<<sgrid(n), kBlock, 0, stream()>>>(n, v, res);
   sync();
 }
 
 void hip_vec_copy(const int n, const double *src, double *dest) {
-  if (n > 0) LAP_HIP(hipMemcpyAsync(dest, src, sizeof(double) * (size_t)n,
+  if (n <= 0 || !check_device("hip_vec_copy", {{src, "src"}, {dest, "dest"}})) return;
+  LAP_HIP(hipMemcpyAsync(dest, src, sizeof(double) * (size_t)n,
                                     hipMemcpyDeviceToDevice, stream()));
   sync();
 }
 
 void hip_vec_zero(const int n, double *vec) {
+  if (n <= 0 || !check_device("hip_vec_zero", {{vec, "vec"}})) return;
   launch_fill(n, 0.0, vec);
   sync();
 }
 
 void hip_vec_set(const int n, double value, double *vec) {
+  if (n <= 0 || !check_device("hip_vec_set", {{vec, "vec"}})) return;
   launch_fill(n, value, vec);
   sync();
 }

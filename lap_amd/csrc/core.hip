@@ -67,6 +67,30 @@ bool is_device_ptr(const void *p) {
          attr.type == hipMemoryTypeUnified;
 }
 
+bool check_device(const char *fn, std::initializer_list<PtrArg> args) {
+  for (const PtrArg &a : args) {
+    if (!is_device_ptr(a.p)) {
+      set_error(LAP_ERR_ARG, "%s: %s (%p) is not device memory", fn, a.name, a.p);
+      return false;
+    }
+    if (a.bytes) {
+      hipDeviceptr_t base = nullptr;
+      size_t size = 0;
+      if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)a.p) == hipSuccess) {
+        const size_t avail = (size_t)((const char *)base + size - (const char *)a.p);
+        if (avail < a.bytes) {
+          set_error(LAP_ERR_ARG, "%s: %s holds %zu bytes, %zu needed", fn, a.name, avail,
+                    a.bytes);
+          return false;
+        }
+      } else {
+        (void)hipGetLastError();  // extent unknown: accept
+      }
+    }
+  }
+  return true;
+}
+
 DevBuf::DevBuf(size_t b) : bytes(b) {
   if (b == 0) b = 16;
   if (!LAP_HIP(hipMalloc(&p, b))) p = nullptr;
@@ -177,14 +201,23 @@ void *mallocForDevice(void *x, int n, int size) {
 
 void memcpyDevice(void *dest, void *src, int n, int size, char *type) {
   const size_t bytes = (size_t)n * (size_t)size;
+  if (bytes == 0) return;
   hipMemcpyKind kind;
-  if (!strcmp(type, "H2D")) kind = hipMemcpyHostToDevice;
-  else if (!strcmp(type, "D2H")) kind = hipMemcpyDeviceToHost;
-  else if (!strcmp(type, "D2D")) kind = hipMemcpyDeviceToDevice;
-  else {
+  bool ok;
+  if (!strcmp(type, "H2D")) {
+    kind = hipMemcpyHostToDevice;
+    ok = check_device("memcpyDevice H2D", {{dest, "dest"}});
+  } else if (!strcmp(type, "D2H")) {
+    kind = hipMemcpyDeviceToHost;
+    ok = check_device("memcpyDevice D2H", {{src, "src"}});
+  } else if (!strcmp(type, "D2D")) {
+    kind = hipMemcpyDeviceToDevice;
+    ok = check_device("memcpyDevice D2D", {{dest, "dest"}, {src, "src"}});
+  } else {
     set_error(LAP_ERR_ARG, "memcpyDevice: unknown direction '%s'", type);
     return;
   }
+  if (!ok) return;
   LAP_HIP(hipMemcpyAsync(dest, src, bytes, kind, stream()));
   sync();
 }

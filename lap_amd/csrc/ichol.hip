@@ -213,6 +213,7 @@ using namespace lap;
 extern "C" {
 
 void initialize_ichol(const int n, const int nnzA, int *ia, int *ja, double *a) {
+  if (n <= 0 || !check_device("initialize_ichol", {{ia, "ia"}, {ja, "ja"}, {a, "a"}})) return;
   ic_factor(n, nnzA, ia, ja, a);
   sync();
 }
@@ -229,7 +230,9 @@ void hip_ichol(const int *ia, const int *ja, double *a, const int nnzA, lap_pdat
               (void *)prec_data->ichol_vals);
     return;
   }
-  double *aux = prec_data->aux_vec1 ? prec_data->aux_vec1 : scratch_vec("ichol_aux", p->n);
+  if (!check_device("hip_ichol", {{x, "x"}, {y, "y"}})) return;
+  double *aux = is_device_ptr(prec_data->aux_vec1) ? prec_data->aux_vec1
+                                                   : scratch_vec("ichol_aux", p->n);
   ic_apply_launch(*p, x, y, aux, nullptr);
   sync();
 }

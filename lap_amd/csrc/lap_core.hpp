@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
 #include <cstdio>
 #include <map>
 #include <memory>
@@ -36,6 +37,17 @@ void sync();  // hipStreamSynchronize(stream()) with error capture
 // True when p is device (or managed) memory of the HIP runtime.  Used to
 // refuse host pointers before any kernel could dereference them.
 bool is_device_ptr(const void *p);
+// Records an error and returns false unless every listed pointer is device
+// memory with at least `bytes` bytes from p to the end of its allocation;
+// every public entry point checks its caller pointers with it before
+// launching anything, so a host pointer or a short buffer is refused, never
+// read by a kernel.
+struct PtrArg {
+  const void *p;
+  const char *name;
+  size_t bytes = 0;  // required extent from p (0: not checked)
+};
+bool check_device(const char *fn, std::initializer_list<PtrArg> args);
 
 // Device buffer owned by the library (RAII over hipMalloc/hipFree).
 struct DevBuf {

@@ -421,7 +421,14 @@ static bool prec_setup(PrecOps &o, int n, int nnzA, const int *ia, const int *ja
   o.A = spmv_plan(n, nnzA, ia, ja);
   if (!o.A) return false;
   auto aux = [&](double *given, const char *tag) -> double * {
-    if (given && is_device_ptr(given)) return given;
+    if (given && is_device_ptr(given)) {
+      hipDeviceptr_t base = nullptr;
+      size_t size = 0;
+      if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)given) == hipSuccess &&
+          (size_t)((const char *)base + size - (const char *)given) >= sizeof(double) * (size_t)n)
+        return given;
+      (void)hipGetLastError();
+    }
     return scratch_vec(tag, (size_t)n);
   };
   o.aux1 = aux(pd->aux_vec1, "prec_aux1");
@@ -429,18 +436,22 @@ static bool prec_setup(PrecOps &o, int n, int nnzA, const int *ia, const int *ja
   o.aux3 = aux(pd->aux_vec3, "prec_aux3");
   const bool need_dr = o.kind == P_JACOBI || o.kind == P_IT_JACOBI || o.kind == P_GS_IT ||
                        o.kind == P_GS_IT2;
-  if (need_dr && !check_dev(pd->d_r, "prec_data->d_r")) return false;
+  if (need_dr && !check_device("preconditioner",
+                               {{pd->d_r, "prec_data->d_r", sizeof(double) * (size_t)n}}))
+    return false;
   if (o.kind == P_IT_JACOBI || o.kind == P_GS_IT || o.kind == P_GS_IT2) {
-    if (!check_dev(pd->lia, "prec_data->lia") || !check_dev(pd->uia, "prec_data->uia") ||
-        !check_dev(pd->la, "prec_data->la") || !check_dev(pd->ua, "prec_data->ua"))
+    if (!check_device("preconditioner", {{pd->lia, "prec_data->lia"}, {pd->lja, "prec_data->lja"},
+                                         {pd->la, "prec_data->la"}, {pd->uia, "prec_data->uia"},
+                                         {pd->uja, "prec_data->uja"}, {pd->ua, "prec_data->ua"}}))
       return false;
     o.Ls = spmv_plan(n, pd->lnnz, pd->lia, pd->lja);
     o.Us = spmv_plan(n, pd->unnz, pd->uia, pd->uja);
     if (!o.Ls || !o.Us) return false;
   }
   if (o.kind == P_GS_STD) {
-    if (!check_dev(pd->lia, "prec_data->lia") || !check_dev(pd->uia, "prec_data->uia") ||
-        !check_dev(pd->la, "prec_data->la") || !check_dev(pd->ua, "prec_data->ua"))
+    if (!check_device("GS_std", {{pd->lia, "prec_data->lia"}, {pd->lja, "prec_data->lja"},
+                                 {pd->la, "prec_data->la"}, {pd->uia, "prec_data->uia"},
+                                 {pd->uja, "prec_data->uja"}, {pd->ua, "prec_data->ua"}}))
       return false;
     o.Lt = tri_plan(n, pd->lnnz, pd->lia, pd->lja, true);
     o.Ut = tri_plan(n, pd->unnz, pd->uia, pd->uja, false);
@@ -582,8 +593,11 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
     set_error(LAP_ERR_ARG, "pcg: bad sizes n=%d nnz=%d maxit=%d", n, nnz, maxit);
     return LAP_ERR_ARG;
   }
-  if (!check_dev(ia, "ia") || !check_dev(ja, "ja") || !check_dev(a, "a") || !check_dev(x, "x") ||
-      !check_dev(b, "b"))
+  if (!check_device("pcg", {{ia, "ia", sizeof(int) * ((size_t)n + 1)},
+                           {ja, "ja", sizeof(int) * (size_t)nnz},
+                           {a, "a", sizeof(double) * (size_t)nnz},
+                           {x, "x", sizeof(double) * (size_t)n},
+                           {b, "b", sizeof(double) * (size_t)n}}))
     return LAP_ERR_ARG;
   PrecOps o;
   if (!prec_setup(o, n, nnz, ia, ja, a, pd)) return lap_last_status();
@@ -775,8 +789,11 @@ void cg(int n, double nnz, int *ia, int *ja, double *a, double *x, double *b, do
 
 double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
                           const double *p, double *q, int reps) {
+  if (n <= 0 || reps < 1 ||
+      !check_device("lap_pcg_spmv_timed", {{ia, "ia"}, {ja, "ja"}, {a, "a"}, {p, "p"}, {q, "q"}}))
+    return -1.0;
   const SpmvPlan *A = spmv_plan(n, nnz, ia, ja);
-  if (!A || reps < 1) return -1.0;
+  if (!A) return -1.0;
   ReduceWs &ws = reduce_ws();
   PcgCtl h{};
   h.rho = 1.0;
@@ -802,6 +819,13 @@ double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const do
 void prec_function(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, double *x,
                    double *y) {
   PrecOps o;
+  if (!prec_data || prec_data->n <= 0) {
+    set_error(LAP_ERR_ARG, "prec_function: missing prec_data");
+    return;
+  }
+  if (!check_device("prec_function",
+                    {{ia, "ia"}, {ja, "ja"}, {a, "a"}, {x, "x"}, {y, "y"}}))
+    return;
   if (!prec_setup(o, prec_data->n, nnzA, ia, ja, a, prec_data)) return;
   prec_launch(o, x, y, nullptr);
   sync();

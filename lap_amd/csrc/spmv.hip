@@ -159,6 +159,9 @@ void analyze_spmv(const int n, const int nnz, int *ia, int *ja, double *a, const
   (void)result;
   if (option && strcmp(option, "A") && strcmp(option, "L") && strcmp(option, "U"))
     fprintf(stderr, "[lapgfx] analyze_spmv: unknown option '%s' (analysed anyway)\n", option);
+  if (n <= 0 || !check_device("analyze_spmv", {{ia, "ia", sizeof(int) * ((size_t)n + 1)},
+                                               {ja, "ja", sizeof(int) * (size_t)nnz}}))
+    return;
   spmv_plan(n, nnz, ia, ja);
 }
 
@@ -166,6 +169,12 @@ void hip_csr_matvec(const int n, const int nnz, const int *ia, const int *ja, co
                     const double *x, double *result, const double *al, const double *bet,
                     const char *kind) {
   (void)kind;  // plans are keyed by the arrays themselves
+  if (n <= 0) return;
+  if (!check_device("hip_csr_matvec",
+                    {{ia, "ia", sizeof(int) * ((size_t)n + 1)}, {ja, "ja", sizeof(int) * (size_t)nnz},
+                     {a, "values", sizeof(double) * (size_t)nnz}, {x, "x", sizeof(double) * (size_t)n},
+                     {result, "result", sizeof(double) * (size_t)n}}))
+    return;
   const SpmvPlan *p = spmv_plan(n, nnz, ia, ja);
   if (!p) return;
   spmv_launch(*p, ia, ja, a, x, result, *al, *bet);
@@ -180,8 +189,11 @@ void csr_matvec(const int n, const int nnz, const int *ia, const int *ja, const 
 
 double lap_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
                       const double *x, double *y, int reps) {
+  if (n <= 0 || reps < 1 ||
+      !check_device("lap_spmv_timed", {{ia, "ia"}, {ja, "ja"}, {a, "a"}, {x, "x"}, {y, "y"}}))
+    return -1.0;
   const SpmvPlan *p = spmv_plan(n, nnz, ia, ja);
-  if (!p || reps < 1) return -1.0;
+  if (!p) return -1.0;
   spmv_launch(*p, ia, ja, a, x, y, 1.0, 0.0);  // warm
   hipEvent_t e0, e1;
   LAP_HIP(hipEventCreate(&e0));

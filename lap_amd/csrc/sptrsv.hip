@@ -24,16 +24,20 @@ __global__ __launch_bounds__(kBlock) void k_tri_level(
   if (t >= cnt) return;
   const int row = rows[t];
   double acc = x[row];
-  double piv = diag ? diag[row] : 0.0;
+  double piv = 0.0;
+  bool in_row = false;
   const int e1 = ia[row + 1];
   for (int e = ia[row]; e < e1; ++e) {
     const int c = ja[e];
     if (c == row) {
       piv = a[e];
+      in_row = true;
       continue;
     }
     acc -= a[e] * y[c];
   }
+  // diag is only passed (and only read) for matrices without in-row pivots
+  if (!in_row && diag) piv = diag[row];
   y[row] = acc / piv;
 }
 
@@ -128,14 +132,28 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
 
 static void tri_entry(int n, int nnz, const int *ia, const int *ja, const double *a,
                       const double *diag, const double *x, double *y, bool lower) {
+  if (n <= 0) return;
+  const char *fn = lower ? "hip_lower_triangular_solve" : "hip_upper_triangular_solve";
+  if (!check_device(fn, {{ia, "ia", sizeof(int) * ((size_t)n + 1)},
+                         {ja, "ja", sizeof(int) * (size_t)nnz},
+                         {a, "values", sizeof(double) * (size_t)nnz},
+                         {x, "x", sizeof(double) * (size_t)n},
+                         {y, "result", sizeof(double) * (size_t)n}}))
+    return;
   const TriPlan *p = tri_plan(n, nnz, ia, ja, lower);
   if (!p) return;
-  if (!p->has_diag && !diag) {
-    set_error(LAP_ERR_STRUCT, "%s triangular solve: a row has no pivot and diagonal is NULL",
-              lower ? "lower" : "upper");
-    return;
+  // With the pivots inside the matrix (the GPU drivers' L+D / D+U layout)
+  // `diagonal` is ignored, as rocSPARSE ignores it (hip_blas.cpp:534-591);
+  // the reference drivers may pass a stale host pointer there
+  // (cg_driver2.c:246-248).
+  const double *dg = nullptr;
+  if (!p->has_diag) {
+    if (!check_device(fn, {{diag, "diagonal (rows without an in-row pivot)",
+                            sizeof(double) * (size_t)n}}))
+      return;
+    dg = diag;
   }
-  tri_solve_launch(*p, ia, ja, a, diag, x, y, nullptr);
+  tri_solve_launch(*p, ia, ja, a, dg, x, y, nullptr);
   sync();
 }
 
@@ -150,6 +168,10 @@ void initialize_and_analyze_L_and_U_solve(const int n, const int nnzL, int *lia,
                                           double *ua) {
   (void)la;
   (void)ua;
+  if (n <= 0) return;
+  if (!check_device("initialize_and_analyze_L_and_U_solve",
+                    {{lia, "lia"}, {lja, "lja"}, {uia, "uia"}, {uja, "uja"}}))
+    return;
   tri_plan(n, nnzL, lia, lja, true);
   tri_plan(n, nnzU, uia, uja, false);
 }

@@ -183,3 +183,61 @@ def test_spmv_rejects_bad_columns():
     bad = CSR(3, np.array([0, 1, 2, 3], np.int32), np.array([0, 5, 2], np.int32), np.ones(3))
     with pytest.raises(L.LapError):
         S.spmv(S.DeviceCSR.from_csr(bad), dev(np.ones(3)))
+
+
+def test_trisolve_ignores_stale_diagonal_with_inrow_pivots(orc):
+    """The reference's GPU driver hands the triangular solves a freed host
+    pointer as `diagonal` (cg_driver2.c:246-248, GS.c:31/48) while the pivots
+    sit inside L+D / D+U: the solve must ignore it, as rocSPARSE does."""
+    import ctypes as C
+    from lap_amd import lib as L
+    lib = L.load()
+    P = PROBLEMS["grid3d"]()
+    x = np.random.default_rng(7).standard_normal(P.n)
+    host_diag = np.full(P.n, 3.0)  # host memory: never to be read by a kernel
+    hp = host_diag.ctypes.data_as(C.c_void_p)
+    for T, lower, ref in ((augment_lower(P), True, orc.lower_solve),
+                          (augment_upper(P), False, orc.upper_solve)):
+        D = S.DeviceCSR.from_csr(T)
+        out = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+        dx = dev(x)
+        torch.cuda.synchronize()
+        fn = lib.hip_lower_triangular_solve if lower else lib.hip_upper_triangular_solve
+        fn(P.n, D.nnz, S.ptr(D.ia), S.ptr(D.ja), S.ptr(D.a), hp, S.ptr(dx), S.ptr(out))
+        L.check()
+        np.testing.assert_array_equal(host(out), ref(T, x))
+
+
+def test_trisolve_refuses_host_diagonal_for_strict_factor():
+    """A strict factor needs `diagonal`; a host pointer there is an error
+    raised before any launch."""
+    import ctypes as C
+    from lap_amd import lib as L
+    lib = L.load()
+    P = PROBLEMS["grid3d"]()
+    Ls, _, _ = split(P)
+    D = S.DeviceCSR.from_csr(Ls)
+    host_diag = np.full(P.n, 3.0)
+    out = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    dx = dev(np.ones(P.n))
+    torch.cuda.synchronize()
+    lib.hip_lower_triangular_solve(P.n, D.nnz, S.ptr(D.ia), S.ptr(D.ja), S.ptr(D.a),
+                                   host_diag.ctypes.data_as(C.c_void_p), S.ptr(dx), S.ptr(out))
+    with pytest.raises(L.LapError):
+        L.check()
+
+
+def test_entry_points_refuse_host_vectors():
+    """Host pointers where device memory is required are refused, not read."""
+    import ctypes as C
+    from lap_amd import lib as L
+    lib = L.load()
+    h = np.ones(100)
+    hp = h.ctypes.data_as(C.c_void_p)
+    d = dev(np.ones(100))
+    lib.hip_axpy(100, C.c_double(1.0), hp, S.ptr(d))
+    with pytest.raises(L.LapError):
+        L.check()
+    lib.hip_dot(100, S.ptr(d), hp)
+    with pytest.raises(L.LapError):
+        L.check()
