@@ -194,6 +194,13 @@ void lap_clear_error(void);
 
 /* Drops every cached analysis plan, factor and work buffer. */
 void lap_release_all(void);
+/* Plans (SpMV row blocks, triangular level sets, IC0 factors, the mixed-
+ * precision copy, batch orders) are cached by the device addresses of the
+ * arrays they were built from.  Call this when an address is about to hold a
+ * different matrix (e.g. after freeing and reallocating): every plan whose
+ * key holds p is dropped.  analyze_spmv and
+ * initialize_and_analyze_L_and_U_solve always rebuild their plans. */
+void lap_forget(const void *p);
 
 /* Result of one PCG solve. */
 typedef struct lap_solve_info {
@@ -226,6 +233,27 @@ int lap_pcg(int n, int nnz, const int *ia, const int *ja, const double *a, doubl
 int lap_pcg_mixed(int n, int nnz, const int *ia, const int *ja, const double *a, double *x,
                   const double *b, double tol, lap_pdata *prec_data, int maxit, double delta,
                   double *history, lap_solve_info *info);
+
+/* Batched Jacobi PCG: nsys independent SPD systems, concatenated into one
+ * block-diagonal CSR (global row/column indices; system s owns rows
+ * [row_off[s], row_off[s+1])) with concatenated x (initial guess in, solution
+ * out), b and d_r = 1/diag.  One persistent launch: workgroups take whole
+ * systems from an atomic queue (largest first) and run each system's PCG
+ * (cg.c recurrence, line_jacobi) with workgroup-level reductions only.
+ * block (256/512/1024) and tpr (threads per row: 1/2/4) select the kernel;
+ * 0/0 uses the configuration found by lap_pcg_batched_tune (or 256/1).
+ * Per-system outputs (host arrays of nsys, each may be NULL): iterations,
+ * flag (0/1/2 as cg()), ||r0|| and the final recursive residual. */
+int lap_pcg_batched(int nsys, const int *row_off, int n_total, int nnz_total, const int *ia,
+                    const int *ja, const double *a, const double *d_r, double *x,
+                    const double *b, double tol, int maxit, int block, int tpr, int *iters,
+                    int *flags, double *res0, double *res, double *gpu_ms);
+/* Autotuner for lap_pcg_batched: times trial_iters iterations of every
+ * (block, tpr) kernel on this batch (x = 0, the caller's x is not touched)
+ * and keeps the fastest for later calls with block = 0. */
+int lap_pcg_batched_tune(int nsys, const int *row_off, int n_total, int nnz_total,
+                         const int *ia, const int *ja, const double *a, const double *d_r,
+                         const double *b, int trial_iters, int *best_block, int *best_tpr);
 
 /* Kernel-level entry points used by benchmarks and tests (device pointers):
  * y = A x with the fused SpMV, launched `reps` times back to back on the

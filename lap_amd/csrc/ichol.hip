@@ -85,6 +85,14 @@ std::map<const double *, IcEntry> g_ic;
 }  // namespace
 
 void release_ic_plans() { g_ic.clear(); }
+void forget_ic_plans(const void *p) {
+  for (auto it = g_ic.begin(); it != g_ic.end();) {
+    if (it->first == p || it->second.ia == p || it->second.ja == p)
+      it = g_ic.erase(it);
+    else
+      ++it;
+  }
+}
 
 const IcPlan *ic_plan_find(const double *vals) {
   auto it = g_ic.find(vals);

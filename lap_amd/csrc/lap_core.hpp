@@ -186,6 +186,10 @@ inline unsigned grid_for(size_t n, int per_thread = 4, int bs = kBlock, unsigned
   return (unsigned)g;
 }
 
+// Drops every cached plan (SpMV, triangular, IC0, mixed, batch) whose key
+// holds the device address p.
+void forget_plans(const void *p);
+
 // ------------------------------------------------------------- SpMV plan
 struct SpmvPlan {
   int n = 0, nnz = 0;

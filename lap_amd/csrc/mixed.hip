@@ -476,6 +476,15 @@ float *to_f32_cached(std::map<const double *, DevBufPtr> &cache, const double *s
 }  // namespace
 
 void release_mix_plans() { g_mix.clear(); }
+void forget_mix_plans(const void *p) {
+  for (auto it = g_mix.begin(); it != g_mix.end();) {
+    const MixKey &k = it->first;
+    if (std::get<0>(k) == p || std::get<1>(k) == p || std::get<2>(k) == p)
+      it = g_mix.erase(it);
+    else
+      ++it;
+  }
+}
 
 struct MixOut {
   int iterations = 0, flag = 2, refinements = 0;

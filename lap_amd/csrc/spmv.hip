@@ -139,7 +139,26 @@ void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *
 void release_tri_plans();
 void release_ic_plans();
 void release_mix_plans();
+void release_batch_plans();
+void forget_tri_plans(const void *p);
+void forget_ic_plans(const void *p);
+void forget_mix_plans(const void *p);
+void forget_batch_plans(const void *p);
+
+void forget_plans(const void *p) {
+  for (auto it = g_plans.begin(); it != g_plans.end();) {
+    if (std::get<0>(it->first) == p || std::get<1>(it->first) == p)
+      it = g_plans.erase(it);
+    else
+      ++it;
+  }
+  forget_tri_plans(p);
+  forget_ic_plans(p);
+  forget_mix_plans(p);
+  forget_batch_plans(p);
+}
 void release_plans() {
+  release_batch_plans();
   release_mix_plans();
   g_plans.clear();
   release_tri_plans();
@@ -162,6 +181,8 @@ void analyze_spmv(const int n, const int nnz, int *ia, int *ja, double *a, const
   if (n <= 0 || !check_device("analyze_spmv", {{ia, "ia", sizeof(int) * ((size_t)n + 1)},
                                                {ja, "ja", sizeof(int) * (size_t)nnz}}))
     return;
+  // an explicit analysis always rebuilds (the arrays may hold a new matrix)
+  g_plans.erase(Key{ia, ja, n, nnz, options().spmv_rows});
   spmv_plan(n, nnz, ia, ja);
 }
 
@@ -185,6 +206,11 @@ void csr_matvec(const int n, const int nnz, const int *ia, const int *ja, const 
                 const double *x, double *result, const double *al, const double *bet,
                 const char *kind) {
   hip_csr_matvec(n, nnz, ia, ja, a, x, result, al, bet, kind);
+}
+
+void lap_forget(const void *p) {
+  sync();
+  forget_plans(p);
 }
 
 double lap_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double *a,

@@ -47,6 +47,14 @@ std::map<Key, std::unique_ptr<TriPlan>> g_plans;
 }  // namespace
 
 void release_tri_plans() { g_plans.clear(); }
+void forget_tri_plans(const void *p) {
+  for (auto it = g_plans.begin(); it != g_plans.end();) {
+    if (std::get<0>(it->first) == p || std::get<1>(it->first) == p)
+      it = g_plans.erase(it);
+    else
+      ++it;
+  }
+}
 
 // Host-side level analysis of a triangular CSR pattern.
 std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
@@ -172,6 +180,8 @@ void initialize_and_analyze_L_and_U_solve(const int n, const int nnzL, int *lia,
   if (!check_device("initialize_and_analyze_L_and_U_solve",
                     {{lia, "lia"}, {lja, "lja"}, {uia, "uia"}, {uja, "uja"}}))
     return;
+  g_plans.erase(Key{lia, lja, n, nnzL, true});  // explicit analysis rebuilds
+  g_plans.erase(Key{uia, uja, n, nnzU, false});
   tri_plan(n, nnzL, lia, lja, true);
   tri_plan(n, nnzU, uia, uja, false);
 }

@@ -66,11 +66,18 @@ _SIGS = {
     "lap_last_error": (C.c_char_p, []),
     "lap_clear_error": (None, []),
     "lap_release_all": (None, []),
+    "lap_forget": (None, [VP]),
     "lap_pcg": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_double, C.POINTER(PData),
                           C.c_int, C.POINTER(C.c_double), C.POINTER(SolveInfo)]),
     "lap_pcg_mixed": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_double,
                                 C.POINTER(PData), C.c_int, C.c_double, C.POINTER(C.c_double),
                                 C.POINTER(SolveInfo)]),
+    "lap_pcg_batched": (C.c_int, [C.c_int, PI, C.c_int, C.c_int, PI, PI, PD, PD, PD, PD,
+                                  C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "lap_pcg_batched_tune": (C.c_int, [C.c_int, PI, C.c_int, C.c_int, PI, PI, PD, PD, PD,
+                                       C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lap_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
     "lap_pcg_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
     "lap_set_option": (C.c_int, [C.c_char_p, C.c_int]),

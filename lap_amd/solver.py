@@ -47,8 +47,18 @@ class DeviceCSR:
     @classmethod
     def from_csr(cls, A: CSR, device: str = DEV) -> "DeviceCSR":
         A.check()
-        return cls(A.n, torch.from_numpy(A.ia).to(device), torch.from_numpy(A.ja).to(device),
-                   torch.from_numpy(np.ascontiguousarray(A.a)).to(device))
+        D = cls(A.n, torch.from_numpy(A.ia).to(device), torch.from_numpy(A.ja).to(device),
+                torch.from_numpy(np.ascontiguousarray(A.a)).to(device))
+        D.forget()
+        return D
+
+    def forget(self) -> None:
+        """The torch allocator may hand these arrays addresses that held
+        another matrix: drop any plan cached for those addresses."""
+        lib = L.load()
+        torch.cuda.synchronize()
+        for t in (self.ia, self.ja, self.a):
+            lib.lap_forget(C.c_void_p(t.data_ptr()))
 
 
 def to_dev(v: np.ndarray) -> torch.Tensor:
@@ -129,6 +139,7 @@ def make_preconditioner(A: DeviceCSR, host: CSR, kind: str, m: int = 1, k: int =
                                                  Ud.nnz, ptr(Ud.ia), ptr(Ud.ja), ptr(Ud.a))
     if kind == "ichol":
         ichol_vals = A.a.clone()
+        lib.lap_forget(ptr(ichol_vals))
         lib.initialize_ichol(n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(ichol_vals))
         keep["ichol_vals"] = ichol_vals
     L.check()
@@ -231,3 +242,71 @@ def pcg_mixed(A: DeviceCSR, b: torch.Tensor, P: Preconditioner, tol: float = 1e-
         raise L.LapError(f"lap_pcg_mixed returned {st}")
     return MixedResult(x, info.iterations, info.flag, hist[: info.iterations + 1].copy(),
                        info.res0, info.true_res, info.gpu_ms, info.refinements)
+
+
+# --------------------------------------------------------------- batching
+
+@dataclass
+class Batch:
+    """Many systems as one block-diagonal device CSR (see lap_pcg_batched)."""
+    nsys: int
+    row_off: torch.Tensor
+    A: DeviceCSR
+    d_r: torch.Tensor
+    b: torch.Tensor
+    sizes: np.ndarray
+
+    @classmethod
+    def from_systems(cls, systems, rhs) -> "Batch":
+        sizes = np.array([S_.n for S_ in systems], dtype=np.int64)
+        row_off = np.zeros(len(systems) + 1, dtype=np.int64)
+        np.cumsum(sizes, out=row_off[1:])
+        nnz = np.array([S_.nnz for S_ in systems], dtype=np.int64)
+        nz_off = np.zeros(len(systems) + 1, dtype=np.int64)
+        np.cumsum(nnz, out=nz_off[1:])
+        if row_off[-1] >= 2**31 or nz_off[-1] >= 2**31:
+            raise L.LapError("batch exceeds int32 CSR")
+        ia = np.concatenate([S_.ia[:-1].astype(np.int64) + o for S_, o in zip(systems, nz_off)]
+                            + [nz_off[-1:]])
+        ja = np.concatenate([S_.ja.astype(np.int64) + o for S_, o in zip(systems, row_off)])
+        a = np.concatenate([S_.a for S_ in systems])
+        big = CSR(int(row_off[-1]), ia.astype(np.int32), ja.astype(np.int32), a)
+        d = diagonal(big)
+        d_r = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0)
+        B = cls(len(systems), torch.from_numpy(row_off.astype(np.int32)).to(DEV),
+                DeviceCSR.from_csr(big), to_dev(d_r), to_dev(np.concatenate(rhs)), sizes)
+        L.load().lap_forget(ptr(B.row_off))
+        return B
+
+    def split(self, v: torch.Tensor):
+        off = self.row_off.cpu().numpy()
+        return [v[off[i]:off[i + 1]] for i in range(self.nsys)]
+
+
+def batched_tune(B: Batch, trial_iters: int = 20):
+    bs, tpr = C.c_int(0), C.c_int(0)
+    _sync()
+    L.load().lap_pcg_batched_tune(B.nsys, ptr(B.row_off), B.A.n, B.A.nnz, ptr(B.A.ia),
+                                  ptr(B.A.ja), ptr(B.A.a), ptr(B.d_r), ptr(B.b), trial_iters,
+                                  C.byref(bs), C.byref(tpr))
+    L.check()
+    return bs.value, tpr.value
+
+
+def pcg_batched(B: Batch, tol: float = 1e-10, maxit: int = 25000, block: int = 0,
+                tpr: int = 0, x0: torch.Tensor | None = None):
+    x = torch.zeros(B.A.n, dtype=torch.float64, device=DEV) if x0 is None else x0.clone()
+    it = np.zeros(B.nsys, dtype=np.int32)
+    fl = np.zeros(B.nsys, dtype=np.int32)
+    r0 = np.zeros(B.nsys)
+    rs = np.zeros(B.nsys)
+    ms = C.c_double(0.0)
+    _sync()
+    L.load().lap_pcg_batched(B.nsys, ptr(B.row_off), B.A.n, B.A.nnz, ptr(B.A.ia), ptr(B.A.ja),
+                             ptr(B.A.a), ptr(B.d_r), ptr(x), ptr(B.b), tol, maxit, block, tpr,
+                             it.ctypes.data_as(C.POINTER(C.c_int)),
+                             fl.ctypes.data_as(C.POINTER(C.c_int)),
+                             r0.ctypes.data_as(C.POINTER(C.c_double)),
+                             rs.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ms))
+    L.check()
+    return x, it, fl, r0, rs, ms.value

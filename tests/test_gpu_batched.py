@@ -1,0 +1,67 @@
+"""Batched Jacobi PCG in one persistent launch (lap_pcg_batched): every
+system agrees with its own oracle solve (iteration count +-1, solution to
+1e-9 relative with the mean removed for the shifted Laplacians), for every
+kernel configuration and the autotuned one; degenerate systems (1 row, zero
+right-hand side) are handled."""
+import numpy as np
+import pytest
+
+from cases import weighted_graph
+from lap_amd import solver as S
+from lap_amd.problems import grid_laplacian, rhs_normal
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(seed=5, count=24):
+    rng = np.random.default_rng(seed)
+    systems, rhs = [], []
+    for k in range(count):
+        side = int(rng.integers(8, 41))
+        A = grid_laplacian([side, side], 1e-8)
+        systems.append(A)
+        rhs.append(rhs_normal(A.n, seed + k))
+    systems.append(grid_laplacian([1], 1e-8))        # 1x1: b = 0 after mean removal
+    rhs.append(np.zeros(1))
+    systems.append(weighted_graph(700, 10, seed))     # unstructured, uneven rows
+    rhs.append(rhs_normal(700, 99))
+    return systems, rhs
+
+
+def centred(v):
+    return v - v.mean()
+
+
+@pytest.mark.parametrize("block,tpr", [(256, 1), (512, 2), (1024, 4), (0, 0)])
+def test_batched_matches_per_system_oracle(orc, block, tpr):
+    print(f"[case] batched block={block} tpr={tpr}", flush=True)
+    systems, rhs = make_batch()
+    B = S.Batch.from_systems(systems, rhs)
+    if block == 0:
+        print(f"[diag] autotuned: {S.batched_tune(B, 10)}", flush=True)
+    x, it, fl, r0, rs, ms = S.pcg_batched(B, tol=1e-10, maxit=5000, block=block, tpr=tpr)
+    xs = [v.cpu().numpy() for v in B.split(x)]
+    worst = 0.0
+    for k, (A, b) in enumerate(zip(systems, rhs)):
+        xo, ito, flo, _ = orc.cg(A, b, "line_jacobi", tol=1e-10, maxit=5000)
+        assert fl[k] == flo == 0, (k, fl[k], flo)
+        assert abs(int(it[k]) - ito) <= 1, (k, it[k], ito)
+        if np.linalg.norm(centred(xo)) > 0:
+            err = np.linalg.norm(centred(xs[k]) - centred(xo)) / np.linalg.norm(centred(xo))
+            worst = max(worst, err)
+    print(f"[diag] batched {block}/{tpr}: {ms:.2f} ms, worst rel err {worst:.2e}", flush=True)
+    assert worst <= 1e-9
+    assert it[-2] == 0 and fl[-2] == 0  # the 1x1 system with b = 0
+
+
+def test_batched_tpr1_rows_are_reference_order(orc):
+    """With one thread per row and maxit-bounded runs the iterates follow the
+    oracle's (only the dot-product order differs)."""
+    systems, rhs = make_batch(seed=9, count=6)
+    B = S.Batch.from_systems(systems, rhs)
+    x, it, fl, _, _, _ = S.pcg_batched(B, tol=1e-30, maxit=12, block=256, tpr=1)
+    for k, (A, b) in enumerate(zip(systems[:-2], rhs[:-2])):
+        xo, ito, _, _ = orc.cg(A, b, "line_jacobi", tol=1e-30, maxit=12)
+        assert it[k] == ito == 13
+        xk = B.split(x)[k].cpu().numpy()
+        assert np.linalg.norm(xk - xo) <= 1e-10 * np.linalg.norm(xo)
