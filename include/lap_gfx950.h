@@ -255,6 +255,26 @@ int lap_pcg_batched_tune(int nsys, const int *row_off, int n_total, int nnz_tota
                          const int *ia, const int *ja, const double *a, const double *d_r,
                          const double *b, int trial_iters, int *best_block, int *best_tpr);
 
+/* Multi-RHS (no reference counterpart; the reference's cg_driver2.c solves
+ * one right-hand side per run): k independent line_jacobi PCG solves of one
+ * matrix, each with cg.c's recurrence and stopping rule, sharing one SpMM
+ * per iteration so the matrix is read once for all k columns.
+ * X and B are n x k row-interleaved (element (i, j) at i*k + j), device
+ * memory; k is a power of two <= 64; d_r = 1/diag(A) as pdata.d_r.
+ * tile: lanes per row group of the SpMM (divides 64 and k, k/tile <= 8),
+ * 0 = pick by timing (kept per matrix and k).  hist (host, may be NULL):
+ * (maxit+2) x k residual norms, row it = iteration it (0 past a column's
+ * own stop).  Per-column outputs
+ * (host arrays of k, each may be NULL) as lap_pcg_batched. */
+int lap_pcg_block(int n, int nnz, const int *ia, const int *ja, const double *a,
+                  const double *d_r, int k, double *X, const double *B, double tol, int maxit,
+                  int tile, double *hist, int *iters, int *flags, double *res0, double *res,
+                  double *gpu_ms);
+/* Y = A X for k row-interleaved columns (the SpMM of lap_pcg_block); each
+ * column equals hip_csr_matvec's result bit for bit. */
+int lap_csr_matmat(int n, int nnz, const int *ia, const int *ja, const double *a, int k,
+                   const double *X, double *Y, int tile);
+
 /* Kernel-level entry points used by benchmarks and tests (device pointers):
  * y = A x with the fused SpMV, launched `reps` times back to back on the
  * library stream; returns the average device milliseconds per launch. */

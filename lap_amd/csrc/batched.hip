@@ -66,7 +66,10 @@ __global__ __launch_bounds__(BS) void k_batched(
   for (;;) {
     if (threadIdx.x == 0) slot = atomicAdd(queue, 1);
     __syncthreads();
-    const int k = slot;
+    // slot and the stop flag below are workgroup-uniform; readfirstlane
+    // makes every branch on them a scalar branch, so no barrier of this loop
+    // can ever be reached by part of a wave
+    const int k = __builtin_amdgcn_readfirstlane(slot);
     __syncthreads();
     if (k >= nsys) return;
     const int sys = order[k];
@@ -84,6 +87,7 @@ __global__ __launch_bounds__(BS) void k_batched(
     double res = res0;
     if (!isfinite(res0)) flag = 2;
     else if (res0 == 0.0) flag = 0;
+    flag = __builtin_amdgcn_readfirstlane(flag);
     const double tolrel = tol * res0;
     // w = d_r r, rho = r.w, p = w
     double rho = 0.0;
@@ -122,6 +126,7 @@ __global__ __launch_bounds__(BS) void k_batched(
       if (!isfinite(res)) flag = 2;
       else if (res < tolrel) flag = 0;
       else if (it > maxit) flag = 1;
+      flag = __builtin_amdgcn_readfirstlane(flag);
       if (flag >= 0) break;
       const double beta = rz / rho;
       rho = rz;
@@ -134,6 +139,11 @@ __global__ __launch_bounds__(BS) void k_batched(
       res0_out[sys] = res0;
       res_out[sys] = res;
     }
+    // Keeps the thread-0 write above and the thread-0 queue pop of the next
+    // round in separate barrier intervals: without it the compiler may
+    // thread the two thread-0 branches into an inner loop that only lane 0
+    // leaves, sending the other lanes back to the slot read (a hang).
+    __syncthreads();
   }
 }
 

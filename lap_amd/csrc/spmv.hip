@@ -140,10 +140,12 @@ void release_tri_plans();
 void release_ic_plans();
 void release_mix_plans();
 void release_batch_plans();
+void release_block_plans();
 void forget_tri_plans(const void *p);
 void forget_ic_plans(const void *p);
 void forget_mix_plans(const void *p);
 void forget_batch_plans(const void *p);
+void forget_block_plans(const void *p);
 
 void forget_plans(const void *p) {
   for (auto it = g_plans.begin(); it != g_plans.end();) {
@@ -156,8 +158,10 @@ void forget_plans(const void *p) {
   forget_ic_plans(p);
   forget_mix_plans(p);
   forget_batch_plans(p);
+  forget_block_plans(p);
 }
 void release_plans() {
+  release_block_plans();
   release_batch_plans();
   release_mix_plans();
   g_plans.clear();

@@ -78,6 +78,11 @@ _SIGS = {
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "lap_pcg_batched_tune": (C.c_int, [C.c_int, PI, C.c_int, C.c_int, PI, PI, PD, PD, PD,
                                        C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "lap_pcg_block": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, PD, C.c_int, PD, PD, C.c_double,
+                                C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "lap_csr_matmat": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, C.c_int, PD, PD, C.c_int]),
     "lap_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
     "lap_pcg_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
     "lap_set_option": (C.c_int, [C.c_char_p, C.c_int]),

@@ -310,3 +310,53 @@ def pcg_batched(B: Batch, tol: float = 1e-10, maxit: int = 25000, block: int = 0
                              rs.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ms))
     L.check()
     return x, it, fl, r0, rs, ms.value
+
+
+def spmm(A: DeviceCSR, X: torch.Tensor, tile: int = 0) -> torch.Tensor:
+    """Y = A X for an (n, k) float64 CUDA tensor X (lap_csr_matmat)."""
+    X = X.contiguous()
+    n, k = X.shape
+    Y = torch.empty_like(X)
+    _sync()
+    L.load().lap_csr_matmat(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a), k, ptr(X), ptr(Y), tile)
+    L.check()
+    return Y
+
+
+@dataclass
+class BlockResult:
+    x: torch.Tensor          # (n, k)
+    iterations: np.ndarray   # per column
+    flags: np.ndarray
+    res0: np.ndarray
+    res: np.ndarray
+    gpu_ms: float
+    history: np.ndarray | None = None  # (iterations.max()+1, k)
+
+
+def pcg_block(A: DeviceCSR, B: torch.Tensor, d_r: torch.Tensor, tol: float = 1e-10,
+              maxit: int = 25000, tile: int = 0, x0: torch.Tensor | None = None,
+              history: bool = False) -> BlockResult:
+    """k independent line_jacobi PCG solves of A X = B sharing each SpMM
+    (lap_pcg_block); B is (n, k) with k a power of two <= 64."""
+    B = B.contiguous()
+    n, k = B.shape
+    X = torch.zeros_like(B) if x0 is None else x0.clone().contiguous()
+    it = np.zeros(k, dtype=np.int32)
+    fl = np.zeros(k, dtype=np.int32)
+    r0 = np.zeros(k)
+    rs = np.zeros(k)
+    hist = np.zeros((maxit + 2, k)) if history else None
+    ms = C.c_double(0.0)
+    _sync()
+    L.load().lap_pcg_block(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a), ptr(d_r), k, ptr(X),
+                           ptr(B), tol, maxit, tile,
+                           hist.ctypes.data_as(C.POINTER(C.c_double)) if history else None,
+                           it.ctypes.data_as(C.POINTER(C.c_int)),
+                           fl.ctypes.data_as(C.POINTER(C.c_int)),
+                           r0.ctypes.data_as(C.POINTER(C.c_double)),
+                           rs.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ms))
+    L.check()
+    if history:
+        hist = hist[: int(it.max()) + 1]
+    return BlockResult(X, it, fl, r0, rs, ms.value, hist)

@@ -1,0 +1,156 @@
+"""Multi-RHS block PCG (lap_pcg_block) and its SpMM (lap_csr_matmat).
+
+Bars:
+  * SpMM: every column bit-identical to the oracle's CSR matvec
+    (simple_csr_matvec order), for every admissible lane-group width;
+  * fixed iteration count (maxit-bounded): every column's iterate and
+    residual history match the oracle's single-RHS line_jacobi PCG to 1e-9
+    relative (only the dot products' summation order differs);
+  * converged: per column, iteration count within +-1 of the oracle's and
+    solution within 1e-9 relative (mean removed, as for the single-RHS
+    solver: the 1e-8-shifted Laplacian's constant vector is a near-null
+    mode); true residual meets the tolerance;
+  * columns that stop at different iterations keep their own x and counts;
+    a zero column stops at iteration 0 with x = x0.
+"""
+import numpy as np
+import pytest
+import torch
+
+from cases import PROBLEMS, star_graph
+from lap_amd import lib as L
+from lap_amd import solver as S
+from lap_amd.problems import diagonal, grid_laplacian, rhs_normal
+
+pytestmark = pytest.mark.gpu
+
+
+def centred(x):
+    return x - x.mean()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def rhs_block(n, k, seed):
+    return np.stack([rhs_normal(n, seed + j) for j in range(k)], axis=1)
+
+
+def d_r_of(Ah):
+    d = diagonal(Ah)
+    return S.to_dev(np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0))
+
+
+@pytest.mark.parametrize("pname", ["grid3d_16", "graph_big", "star"])
+@pytest.mark.parametrize("k", [1, 4, 16, 64])
+def test_spmm_columns_bitwise(orc, pname, k):
+    print(f"[case] spmm problem={pname} k={k}", flush=True)
+    Ah = star_graph(3000) if pname == "star" else PROBLEMS[pname]()
+    A = S.DeviceCSR.from_csr(Ah)
+    X = np.random.default_rng(k).standard_normal((Ah.n, k))
+    Xd = S.to_dev(X)
+    want = np.stack([orc.matvec(Ah, X[:, j].copy()) for j in range(k)], axis=1)
+    for W in (1, 2, 4, 8, 16, 32, 64):
+        if k % W or k // W > 8:
+            continue
+        Y = S.spmm(A, Xd, tile=W).cpu().numpy()
+        assert np.array_equal(Y, want), (W, np.abs(Y - want).max())
+    Y = S.spmm(A, Xd).cpu().numpy()  # timed choice
+    assert np.array_equal(Y, want)
+
+
+@pytest.mark.parametrize("pname", ["grid3d_16", "graph_big"])
+def test_block_fixed_iterations_vs_oracle(orc, pname):
+    print(f"[case] block fixed-iter problem={pname}", flush=True)
+    Ah = PROBLEMS[pname]()
+    k, maxit = 8, 25
+    Bh = rhs_block(Ah.n, k, 40)
+    A = S.DeviceCSR.from_csr(Ah)
+    res = S.pcg_block(A, S.to_dev(Bh), d_r_of(Ah), tol=1e-30, maxit=maxit, history=True)
+    X = res.x.cpu().numpy()
+    for j in range(k):
+        xo, ito, flo, ho = orc.cg(Ah, Bh[:, j].copy(), "line_jacobi", tol=1e-30, maxit=maxit)
+        assert (int(res.iterations[j]), int(res.flags[j])) == (ito, flo) == (maxit + 1, 1)
+        np.testing.assert_allclose(res.history[:, j], ho, rtol=1e-9, atol=1e-12 * ho[0])
+        assert rel(X[:, j], xo) <= 1e-9, j
+
+
+@pytest.mark.parametrize("tile", [0, 8, 64])
+def test_block_converged_vs_oracle(orc, tile):
+    """Columns of different difficulty stop at different iterations."""
+    print(f"[case] block converged tile={tile}", flush=True)
+    Ah = grid_laplacian([20, 18, 16], 1e-8)
+    k = 64
+    Bh = rhs_block(Ah.n, k, 100)
+    # smooth columns converge in far fewer iterations than random ones
+    xs = np.linspace(0.0, 1.0, Ah.n)
+    for j in range(0, k, 8):
+        Bh[:, j] = np.cos(np.pi * xs) - np.cos(np.pi * xs).mean()
+    Bh[:, 5] = 0.0
+    A = S.DeviceCSR.from_csr(Ah)
+    res = S.pcg_block(A, S.to_dev(Bh), d_r_of(Ah), tol=1e-10, maxit=5000, tile=tile)
+    X = res.x.cpu().numpy()
+    assert res.iterations[5] == 0 and res.flags[5] == 0 and not X[:, 5].any()
+    assert len(set(res.iterations.tolist())) > 2  # columns stopped at different points
+    worst = 0.0
+    for j in range(k):
+        if j == 5:
+            continue
+        b = Bh[:, j].copy()
+        xo, ito, flo, _ = orc.cg(Ah, b, "line_jacobi", tol=1e-10, maxit=5000)
+        assert res.flags[j] == flo == 0
+        assert abs(int(res.iterations[j]) - ito) <= 1, (j, res.iterations[j], ito)
+        worst = max(worst, rel(centred(X[:, j]), centred(xo)))
+        true = np.linalg.norm(b - orc.matvec(Ah, X[:, j].copy()))
+        assert true <= 1e-10 * res.res0[j] * 1.5
+    print(f"[diag] block converged: iterations {res.iterations.min()}..{res.iterations.max()}, "
+          f"worst x rel dev {worst:.2e}, {res.gpu_ms:.1f} ms", flush=True)
+    assert worst <= 1e-9
+
+
+def test_block_matches_single_rhs_solver():
+    """Each column against the single-RHS device solver (same recurrence)."""
+    Ah = PROBLEMS["graph_big"]()
+    k = 4
+    Bh = rhs_block(Ah.n, k, 7)
+    A = S.DeviceCSR.from_csr(Ah)
+    res = S.pcg_block(A, S.to_dev(Bh), d_r_of(Ah), tol=1e-10, maxit=5000)
+    P = S.make_preconditioner(A, Ah, "line_jacobi")
+    for j in range(k):
+        one = S.pcg(A, S.to_dev(Bh[:, j].copy()), P, tol=1e-10, maxit=5000)
+        assert abs(one.iterations - int(res.iterations[j])) <= 1
+        assert rel(res.x[:, j].cpu().numpy(), one.x.cpu().numpy()) <= 1e-8
+
+
+def test_block_initial_guess_and_repeat(orc):
+    Ah = PROBLEMS["grid3d_16"]()
+    k = 2
+    Bh = rhs_block(Ah.n, k, 3)
+    A = S.DeviceCSR.from_csr(Ah)
+    d_r = d_r_of(Ah)
+    r1 = S.pcg_block(A, S.to_dev(Bh), d_r, tol=1e-10)
+    r2 = S.pcg_block(A, S.to_dev(Bh), d_r, tol=1e-10)
+    assert torch.equal(r1.x, r2.x)  # deterministic
+    # an initial guess enters through r0 = b - A x0
+    x0 = np.random.default_rng(9).standard_normal((Ah.n, k))
+    r3 = S.pcg_block(A, S.to_dev(Bh), d_r, tol=1e-10, x0=S.to_dev(x0))
+    assert (r3.flags == 0).all()
+    X1, X3 = r1.x.cpu().numpy(), r3.x.cpu().numpy()
+    for j in range(k):
+        r0 = np.linalg.norm(Bh[:, j] - orc.matvec(Ah, x0[:, j].copy()))
+        assert abs(r3.res0[j] - r0) <= 1e-12 * r0
+        # both solve the system; x0's constant component stays (near-null mode)
+        assert rel(centred(X3[:, j]), centred(X1[:, j])) <= 1e-6
+
+
+def test_block_rejects_bad_arguments():
+    Ah = grid_laplacian([8, 8], 1e-8)
+    A = S.DeviceCSR.from_csr(Ah)
+    d_r = d_r_of(Ah)
+    with pytest.raises(L.LapError):
+        S.pcg_block(A, S.to_dev(rhs_block(Ah.n, 3, 1)), d_r)  # k not a power of two
+    with pytest.raises(L.LapError):
+        S.pcg_block(A, S.to_dev(rhs_block(Ah.n, 4, 1)), d_r, tile=8)  # tile does not divide k
+    with pytest.raises(L.LapError):
+        S.spmm(A, S.to_dev(rhs_block(Ah.n, 128, 1)))  # k > 64
