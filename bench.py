@@ -77,6 +77,128 @@ def cpu_baseline(Ah, b, prec: str, tol: float, gpu_iters: int, sample_iters: int
     }
 
 
+def block_iteration_bytes(n: int, nnz: int, k: int) -> int:
+    """Block CG with k right-hand sides: one SpMM (matrix read once) plus
+    104 B per row and column of block-vector traffic."""
+    return 12 * nnz + 4 * (n + 1) + 104 * k * n
+
+
+def bench_block(S, torch, np, side: int = 128, k: int = 64, tol: float = 1e-10) -> dict:
+    """k-RHS block PCG (line_jacobi) on side^3 + 1e-8 I against k separate
+    single-RHS solves of the same columns."""
+    from lap_amd.problems import diagonal, grid_laplacian
+    Ah = grid_laplacian([side] * 3, 1e-8)
+    A = S.DeviceCSR.from_csr(Ah)
+    d_r = S.to_dev(1.0 / diagonal(Ah))
+    Bh = np.random.default_rng(12345).standard_normal((Ah.n, k))
+    Bh -= Bh.mean(axis=0)
+    B = S.to_dev(Bh)
+    S.pcg_block(A, B, d_r, tol=tol)  # warm-up, picks the SpMM tile
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = S.pcg_block(A, B, d_r, tol=tol)
+    wall = time.perf_counter() - t0
+    P = S.make_preconditioner(A, Ah, "line_jacobi")
+    cols = [B[:, j].contiguous() for j in range(k)]
+    S.pcg(A, cols[0], P, tol=tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    single_it = 0
+    for c in cols:
+        single_it += S.pcg(A, c, P, tol=tol).iterations
+    seq = time.perf_counter() - t0
+    its = int(res.iterations.max())
+    return {
+        "workload": f"{side}^3 7-point Laplacian + 1e-8*I, {k} right-hand sides N(0,1) "
+                    f"mean-subtracted, line_jacobi, tol {tol:g}",
+        "value": wall, "unit": "s", "device_ms": res.gpu_ms,
+        "iterations_min_max": [int(res.iterations.min()), its],
+        "flags": sorted(set(res.flags.tolist())),
+        "sequential_single_rhs_s": seq, "sequential_iterations": single_it,
+        "speedup_vs_sequential": seq / wall,
+        "iteration_gbs": block_iteration_bytes(A.n, A.nnz, k) * its / (res.gpu_ms * 1e-3) / 1e9,
+    }
+
+
+def bench_batched(S, torch, np, count: int = 256, tol: float = 1e-10) -> dict:
+    """count 2D grid Laplacians (side ~ U{32..448}) + 1e-8 I in one persistent
+    launch, autotuned, against sequential single-system solves of 16 of them
+    (scaled to count)."""
+    from lap_amd.problems import grid_laplacian, rhs_normal
+    rng = np.random.default_rng(12345)
+    sides = rng.integers(32, 449, count)
+    systems = [grid_laplacian([int(s), int(s)], 1e-8) for s in sides]
+    rhs = [rhs_normal(Ah.n, 1000 + i) for i, Ah in enumerate(systems)]
+    Bt = S.Batch.from_systems(systems, rhs)
+    cfg = S.batched_tune(Bt, 20)
+    S.pcg_batched(Bt, tol=tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, it, fl, _, _, ms = S.pcg_batched(Bt, tol=tol)
+    wall = time.perf_counter() - t0
+    sel = list(range(0, count, max(1, count // 16)))
+    seq_ms = 0.0
+    for i in sel:
+        A = S.DeviceCSR.from_csr(systems[i])
+        P = S.make_preconditioner(A, systems[i], "line_jacobi")
+        b = S.to_dev(rhs[i])
+        S.pcg(A, b, P, tol=tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S.pcg(A, b, P, tol=tol)
+        seq_ms += (time.perf_counter() - t0) * 1e3
+    seq_est = seq_ms * count / len(sel) / 1e3
+    return {
+        "workload": f"{count} 2D 5-point grids, side U{{32..448}} seed 12345, + 1e-8*I, "
+                    f"b~N(0,1) mean-subtracted, line_jacobi, tol {tol:g}",
+        "value": wall, "unit": "s", "device_ms": ms, "rows": int(Bt.A.n),
+        "config": {"block": cfg[0], "threads_per_row": cfg[1]},
+        "iterations_min_max": [int(it.min()), int(it.max())],
+        "flags": sorted(set(fl.tolist())),
+        "sequential_single_solves_s_est": seq_est,
+        "sequential_sample": f"{len(sel)} of {count} systems solved one by one, scaled",
+        "speedup_vs_sequential": seq_est / wall,
+    }
+
+
+def bench_adjoint(torch, np, n: int = 1_000_000, tol: float = 1e-10) -> dict:
+    """Forward + adjoint solve of L_w x = b on a random geometric graph (n
+    points U[0,1]^2, r = sqrt(10/(pi n)), w = exp(-d^2/sigma^2), sigma = r/2)
+    + 1e-4 I, loss 0.5||x - x*||^2 with x* the solution for w*exp(0.1 xi)."""
+    from lap_amd import graph as GR
+    from lap_amd.problems import random_geometric_graph
+    g = random_geometric_graph(n, 12345)
+    rng = np.random.default_rng(12345)
+    G = GR.GraphLaplacian(g.n, g.src, g.dst, 1e-4)
+    b = torch.from_numpy(rng.standard_normal(n)).cuda()
+    wstar = torch.from_numpy(g.w * np.exp(0.1 * rng.standard_normal(g.m))).cuda()
+    xstar = GR.laplacian_solve(G, wstar, b, tol=tol).detach()
+
+    def step():
+        w = torch.from_numpy(g.w).cuda().requires_grad_(True)
+        bb = b.clone().requires_grad_(True)
+        x = GR.laplacian_solve(G, w, bb, tol=tol)
+        loss = 0.5 * ((x - xstar) ** 2).sum()
+        loss.backward()
+        return w.grad, bb.grad
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gw, gb = step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {
+        "workload": f"random geometric graph n={n} m={g.m} + 1e-4*I, line_jacobi, tol {tol:g}, "
+                    "loss 0.5||x - x*||^2, dl/dw and dl/db",
+        "value": wall, "unit": "s",
+        "forward_iterations": G.last_forward.iterations,
+        "adjoint_iterations": G.last_adjoint.iterations,
+        "forward_device_ms": G.last_forward.gpu_ms, "adjoint_device_ms": G.last_adjoint.gpu_ms,
+        "grad_w_norm": float(torch.linalg.norm(gw)), "grad_b_norm": float(torch.linalg.norm(gb)),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +211,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the multi-RHS, batched and adjoint measurements")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -199,6 +323,18 @@ def main() -> int:
         }
         log(f"mixed: {m.iterations} iterations, {m.refinements} refinements, wall "
             f"{mwall * 1e3:.1f} ms, speedup {wall / mwall:.2f}x, err vs fp64 {err:.1e}")
+    if not args.no_extras:
+        for name, fn in (("block", lambda: bench_block(S, torch, np)),
+                         ("batched", lambda: bench_batched(S, torch, np)),
+                         ("adjoint", lambda: bench_adjoint(torch, np))):
+            t0 = time.perf_counter()
+            try:
+                out[name] = fn()
+                log(f"{name}: {out[name]['value'] * 1e3:.1f} ms "
+                    f"(measured in {time.perf_counter() - t0:.1f}s)")
+            except Exception as exc:  # reported, never hides the main line
+                out[name] = {"error": f"{type(exc).__name__}: {exc}"}
+                log(f"{name} failed: {exc}")
     if not args.no_cpu:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(Ah, b_h, args.prec, args.tol, its, args.cpu_iters)

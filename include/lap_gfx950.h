@@ -275,6 +275,33 @@ int lap_pcg_block(int n, int nnz, const int *ia, const int *ja, const double *a,
 int lap_csr_matmat(int n, int nnz, const int *ia, const int *ja, const double *a, int k,
                    const double *X, double *Y, int tile);
 
+/* Differentiation (no reference counterpart; the reference builds an
+ * unweighted Laplacian once on the host, io_utils.c:80-130, :364-420).
+ * L(w) = sum_e w_e (u_s - u_d)(u_s - u_d)^T + shift*I over the undirected
+ * edges e = (src[e], dst[e]) (no self loops, no duplicates), all device
+ * arrays.
+ * lap_graph_pattern: CSR pattern of L (n + 2m entries, rows sorted, the
+ * diagonal always present) with epos[2e], epos[2e+1] the positions of
+ * (s, d) and (d, s) and dpos[i] that of (i, i).  Host-side setup.
+ * lap_graph_values: a = values of L(w) on that pattern (off-diagonals -w_e,
+ * diagonal = the row's weights summed in CSR order, plus shift), and
+ * d_r = 1/diag when d_r is not NULL (the line_jacobi operand).
+ * lap_edge_grad: grad_w[e] = -(lam[s] - lam[d]) (x[s] - x[d]).
+ * lap_pcg_adjoint: lam = L^{-1} grad_x by lap_pcg from lam = 0 with the
+ * forward solve's preconditioner (prec_data), then lap_edge_grad when
+ * grad_w is not NULL; dl/db = lam.  Returns LAP_ERR_NUMERIC if the adjoint
+ * solve does not converge. */
+int lap_graph_pattern(int n, int m, const int *src, const int *dst, int *ia, int *ja, int *epos,
+                      int *dpos);
+int lap_graph_values(int n, int m, const int *ia, const int *ja, const int *epos,
+                     const int *dpos, const double *w, double shift, double *a, double *d_r);
+int lap_edge_grad(int n, int m, const int *src, const int *dst, const double *lam,
+                  const double *x, double *grad_w);
+int lap_pcg_adjoint(int n, int nnz, const int *ia, const int *ja, const double *a,
+                    lap_pdata *prec_data, const double *grad_x, double *lam, double tol,
+                    int maxit, int m, const int *src, const int *dst, const double *x,
+                    double *grad_w, double *history, lap_solve_info *info);
+
 /* Kernel-level entry points used by benchmarks and tests (device pointers):
  * y = A x with the fused SpMV, launched `reps` times back to back on the
  * library stream; returns the average device milliseconds per launch. */

@@ -82,6 +82,12 @@ _SIGS = {
                                 C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "lap_graph_pattern": (C.c_int, [C.c_int, C.c_int, PI, PI, PI, PI, PI, PI]),
+    "lap_graph_values": (C.c_int, [C.c_int, C.c_int, PI, PI, PI, PI, PD, C.c_double, PD, PD]),
+    "lap_edge_grad": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, PD, PD]),
+    "lap_pcg_adjoint": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, C.POINTER(PData), PD, PD,
+                                  C.c_double, C.c_int, C.c_int, PI, PI, PD, PD,
+                                  C.POINTER(C.c_double), C.POINTER(SolveInfo)]),
     "lap_csr_matmat": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, C.c_int, PD, PD, C.c_int]),
     "lap_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
     "lap_pcg_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),

@@ -99,6 +99,58 @@ def rhs_uniform(n: int, seed: int = 12345) -> np.ndarray:
     return b - b.mean()
 
 
+@dataclass
+class EdgeGraph:
+    """Undirected weighted graph: edges (src[e], dst[e]) with src < dst."""
+    n: int
+    src: np.ndarray  # int32
+    dst: np.ndarray  # int32
+    w: np.ndarray    # float64
+    points: np.ndarray | None = None
+
+    @property
+    def m(self) -> int:
+        return int(self.src.shape[0])
+
+
+def random_geometric_graph(n: int, seed: int = 12345, radius: float | None = None) -> EdgeGraph:
+    """n points U[0,1]^2, an edge between points closer than r =
+    sqrt(10/(pi n)) (mean degree ~10), weight exp(-d^2/sigma^2), sigma = r/2."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n, 2))
+    r = float(np.sqrt(10.0 / (np.pi * n))) if radius is None else float(radius)
+    pairs = cKDTree(pts).query_pairs(r, output_type="ndarray")
+    if pairs.shape[0]:
+        pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+    src = pairs[:, 0].astype(np.int32)
+    dst = pairs[:, 1].astype(np.int32)
+    d2 = ((pts[src] - pts[dst]) ** 2).sum(axis=1)
+    sigma = r / 2.0
+    return EdgeGraph(n, src, dst, np.exp(-d2 / sigma**2), pts)
+
+
+def graph_csr(g: EdgeGraph, w: np.ndarray, shift: float) -> CSR:
+    """Host CSR of L(w) = D - W + shift*I (row sums in edge order; for
+    reference and dense checks, not bitwise with the device assembly)."""
+    deg = np.bincount(g.src, w, g.n) + np.bincount(g.dst, w, g.n)
+    rows = np.concatenate([g.dst, np.arange(g.n)])
+    cols = np.concatenate([g.src, np.arange(g.n)])
+    vals = np.concatenate([-w, deg + shift])
+    return from_coo(g.n, rows, cols, vals, symmetric=True)
+
+
+def graph_dense(g: EdgeGraph, w: np.ndarray, shift: float) -> np.ndarray:
+    """Dense L(w) = sum_e w_e (u_s - u_d)(u_s - u_d)^T + shift*I."""
+    Ld = np.zeros((g.n, g.n))
+    np.add.at(Ld, (g.src, g.dst), -w)
+    np.add.at(Ld, (g.dst, g.src), -w)
+    np.add.at(Ld, (g.src, g.src), w)
+    np.add.at(Ld, (g.dst, g.dst), w)
+    Ld[np.diag_indices(g.n)] += shift
+    return Ld
+
+
 def diagonal(A: CSR) -> np.ndarray:
     rows = A.row_ids()
     d = np.zeros(A.n)

@@ -32,6 +32,11 @@
  *       for an eigenvector RHS.
  *   orc_omp_*         same arithmetic as the serial path except the dot
  *       product's summation order; checked to agree within 1e-12.
+ *   orc_graph_values  no reference counterpart: row sums L*1 = shift*1,
+ *       symmetry, and a dense numpy assembly of sum_e w_e (u_s-u_d)(u_s-u_d)^T.
+ *   orc_edge_grad     no reference counterpart: the gradient it gives with
+ *       dense FP64 solves matches central finite differences of the loss
+ *       (numpy, tests/test_oracle.py).
  */
 #include "lap_oracle.h"
 
@@ -374,4 +379,25 @@ void orc_omp_cg(int n, const int *ia, const int *ja, const double *a, double *x,
                 int *flag, double *hist, int spmv_threads, int dot_threads) {
   cg_body(n, ia, ja, a, x, b, tol, p, maxit, it, flag, hist, 1, dot_omp, mv_omp,
           spmv_threads, dot_threads);
+}
+
+/* ------------------------------------------- weighted graph Laplacian */
+
+void orc_graph_values(int n, int m, const int *ia, const int *ja, const int *epos,
+                      const int *dpos, const double *w, double shift, double *a) {
+  for (int e = 0; e < m; ++e) {
+    a[epos[2 * e]] = -w[e];
+    a[epos[2 * e + 1]] = -w[e];
+  }
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int p = ia[i]; p < ia[i + 1]; ++p)
+      if (ja[p] != i) s += -a[p];
+    a[dpos[i]] = s + shift;
+  }
+}
+
+void orc_edge_grad(int m, const int *src, const int *dst, const double *lam, const double *x,
+                   double *g) {
+  for (int e = 0; e < m; ++e) g[e] = -((lam[src[e]] - lam[dst[e]]) * (x[src[e]] - x[dst[e]]));
 }

@@ -98,6 +98,20 @@ void orc_omp_csr_matvec(int n, const int *ia, const int *ja, const double *a,
                         const double *x, double *y, double alpha, double beta,
                         int threads);
 
+/* ---- weighted graph Laplacian and its adjoint gradient --------------- */
+/* No reference counterpart (the reference builds an unweighted Laplacian,
+ * io_utils.c:80-130, :364-420); restated from the definition
+ * L(w) = sum_e w_e (u_s - u_d)(u_s - u_d)^T + shift*I.  On a CSR pattern
+ * with epos/dpos maps: a[epos[2e]] = a[epos[2e+1]] = -w[e]; the diagonal of
+ * row i = (sum of -a over the row's off-diagonal entries, CSR order) +
+ * shift. */
+void orc_graph_values(int n, int m, const int *ia, const int *ja, const int *epos,
+                      const int *dpos, const double *w, double shift, double *a);
+/* g[e] = -(lam[s] - lam[d]) * (x[s] - x[d]): dl/dw_e for L(w) x = b with
+ * adjoint L(w) lam = dl/dx. */
+void orc_edge_grad(int m, const int *src, const int *dst, const double *lam, const double *x,
+                   double *g);
+
 #ifdef __cplusplus
 }
 #endif

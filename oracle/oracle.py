@@ -85,6 +85,8 @@ class Oracle:
                              C.c_int, PI, PI, PD, C.c_int]
         L.orc_omp_cg.argtypes = [C.c_int, PI, PI, PD, PD, PD, C.c_double, C.POINTER(OrcPrec),
                                  C.c_int, PI, PI, PD, C.c_int, C.c_int]
+        L.orc_graph_values.argtypes = [C.c_int, C.c_int, PI, PI, PI, PI, PD, C.c_double, PD]
+        L.orc_edge_grad.argtypes = [C.c_int, PI, PI, PD, PD, PD]
 
     # -- kernels --------------------------------------------------------
     def dot(self, v, w):
@@ -94,6 +96,18 @@ class Oracle:
         y = np.zeros(A.n) if y is None else y.copy()
         self.lib.orc_csr_matvec(A.n, ip(A.ia), ip(A.ja), dp(A.a), dp(x), dp(y), alpha, beta)
         return y
+
+    def graph_values(self, n, ia, ja, epos, dpos, w, shift):
+        """Values of L(w) on a graph pattern (orc_graph_values)."""
+        a = np.zeros(ja.shape[0])
+        self.lib.orc_graph_values(n, w.shape[0], ip(ia), ip(ja), ip(epos), ip(dpos), dp(w),
+                                  shift, dp(a))
+        return a
+
+    def edge_grad(self, src, dst, lam, x):
+        g = np.zeros(src.shape[0])
+        self.lib.orc_edge_grad(src.shape[0], ip(src), ip(dst), dp(lam), dp(x), dp(g))
+        return g
 
     def lower_solve(self, T, x, diag=None):
         y = np.zeros(T.n)

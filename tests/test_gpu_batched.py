@@ -1,8 +1,15 @@
 """Batched Jacobi PCG in one persistent launch (lap_pcg_batched): every
-system agrees with its own oracle solve (iteration count +-1, solution to
-1e-9 relative with the mean removed for the shifted Laplacians), for every
-kernel configuration and the autotuned one; degenerate systems (1 row, zero
-right-hand side) are handled."""
+system agrees with its own oracle solve (solution to 1e-9 relative with the
+mean removed for the shifted Laplacians), for every kernel configuration and
+the autotuned one; degenerate systems (1 row, zero right-hand side) are
+handled.
+
+Iteration counts: within +-1 of the reference's serial path or of its OpenMP
+path (dot products summed by 8 threads).  On the smallest grids (8x8 + 1e-8 I)
+the residual is not monotone near the tolerance (it rises from 6e-10 to 3e-8
+relative before the near-null mode is resolved), so the exit iteration depends
+on the dot products' summation order: the reference's own serial and OpenMP
+builds stop at 36 and 34 iterations there."""
 import numpy as np
 import pytest
 
@@ -44,8 +51,10 @@ def test_batched_matches_per_system_oracle(orc, block, tpr):
     worst = 0.0
     for k, (A, b) in enumerate(zip(systems, rhs)):
         xo, ito, flo, _ = orc.cg(A, b, "line_jacobi", tol=1e-10, maxit=5000)
+        _, ito_omp, _, _ = orc.cg(A, b, "line_jacobi", tol=1e-10, maxit=5000, omp=(16, 8))
         assert fl[k] == flo == 0, (k, fl[k], flo)
-        assert abs(int(it[k]) - ito) <= 1, (k, it[k], ito)
+        assert min(abs(int(it[k]) - ito), abs(int(it[k]) - ito_omp)) <= 1, \
+            (k, it[k], ito, ito_omp)
         if np.linalg.norm(centred(xo)) > 0:
             err = np.linalg.norm(centred(xs[k]) - centred(xo)) / np.linalg.norm(centred(xo))
             worst = max(worst, err)

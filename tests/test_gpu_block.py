@@ -99,8 +99,12 @@ def test_block_converged_vs_oracle(orc, tile):
             continue
         b = Bh[:, j].copy()
         xo, ito, flo, _ = orc.cg(Ah, b, "line_jacobi", tol=1e-10, maxit=5000)
+        _, ito_omp, _, _ = orc.cg(Ah, b, "line_jacobi", tol=1e-10, maxit=5000, omp=(16, 8))
         assert res.flags[j] == flo == 0
-        assert abs(int(res.iterations[j]) - ito) <= 1, (j, res.iterations[j], ito)
+        # +-1 of the reference's serial or OpenMP build (their exit iterations
+        # can differ where the residual is not monotone near the tolerance)
+        assert min(abs(int(res.iterations[j]) - ito), abs(int(res.iterations[j]) - ito_omp)) <= 1, \
+            (j, res.iterations[j], ito, ito_omp)
         worst = max(worst, rel(centred(X[:, j]), centred(xo)))
         true = np.linalg.norm(b - orc.matvec(Ah, X[:, j].copy()))
         assert true <= 1e-10 * res.res0[j] * 1.5
