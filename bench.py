@@ -39,6 +39,20 @@ def jacobi_iteration_bytes(n: int, nnz: int) -> int:
     return spmv_bytes(n, nnz) + 88 * n
 
 
+def measured_traffic(size: int):
+    """HBM bytes per launch of the fused SpMV on size^3 as measured by
+    rocprofv3 FETCH_SIZE + WRITE_SIZE (each scaled by the counter's reading on
+    a 1 GiB copy), from the committed profile summary; None when absent."""
+    path = os.path.join(ROOT, "profiles", f"k_spmv_pq_{size}cubed_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(Ah, b, prec: str, tol: float, gpu_iters: int, sample_iters: int):
     """The reference's OpenMP cg() (compiled from its sources into
     oracle/_ref/) timed on this host for `sample_iters` iterations of the same
@@ -199,6 +213,29 @@ def bench_adjoint(torch, np, n: int = 1_000_000, tol: float = 1e-10) -> dict:
     }
 
 
+def pmc_probe(S, torch, size: int, reps: int) -> int:
+    """Launches for rocprofv3 --pmc: 4 elementwise copies (dst = src * 1) of a
+    1 GiB float64 buffer (read 1 GiB, write 1 GiB each: the calibration of
+    FETCH_SIZE and WRITE_SIZE) and `reps` launches of the fused SpMV on the
+    size^3 Laplacian."""
+    from lap_amd.problems import grid_laplacian
+    src = torch.randn(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log("copy calibration: 4 x 1 GiB")
+    del src, dst
+    Ah = grid_laplacian([size] * 3, 1e-8)
+    A = S.DeviceCSR.from_csr(Ah)
+    x = torch.randn(A.n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    ms = S.spmv_timed(A, x, y, reps, fused=True)
+    log(f"fused SpMV x{reps}: {ms:.4f} ms per launch, algorithmic bytes "
+        f"{spmv_bytes(A.n, A.nnz)}")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +250,9 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the multi-RHS, batched and adjoint measurements")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only run the kernels the HBM-traffic counters are read on: a 1 GiB "
+                         "device copy (calibration) and the fused SpMV, then exit")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -227,6 +267,8 @@ def main() -> int:
     from lap_amd.problems import grid_laplacian, rhs_normal
 
     torch.cuda.set_device(0)
+    if args.pmc_probe:
+        return pmc_probe(S, torch, args.size, args.spmv_reps)
     t0 = time.perf_counter()
     Ah = grid_laplacian([args.size] * 3, 1e-8)
     b_h = rhs_normal(Ah.n, 12345)
@@ -291,7 +333,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.size),
             "algorithmic_bytes": bytes_spmv,
             "avg_ms": spmv_ms,
         },

@@ -261,8 +261,9 @@ int lap_pcg_batched_tune(int nsys, const int *row_off, int n_total, int nnz_tota
  * per iteration so the matrix is read once for all k columns.
  * X and B are n x k row-interleaved (element (i, j) at i*k + j), device
  * memory; k is a power of two <= 64; d_r = 1/diag(A) as pdata.d_r.
- * tile: lanes per row group of the SpMM (divides 64 and k, k/tile <= 8),
- * 0 = pick by timing (kept per matrix and k).  hist (host, may be NULL):
+ * tile: rows per lane group per step of the SpMM (1, 2 or 4; one lane per
+ * column, 64/k row groups per wave), 0 = pick by timing (kept per matrix
+ * and k).  hist (host, may be NULL):
  * (maxit+2) x k residual norms, row it = iteration it (0 past a column's
  * own stop).  Per-column outputs
  * (host arrays of k, each may be NULL) as lap_pcg_batched. */

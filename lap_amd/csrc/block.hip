@@ -8,12 +8,13 @@
 // i*k + j), so a row of P is one contiguous 8k-byte run and a CSR column
 // index addresses a whole run.
 //
-// SpMM: lanes form groups of W consecutive lanes (W | 64, W | k, k/W <= 8);
-// a group walks rows and, for each row, sums a[e] * P[ja[e], c] over the
-// row's nonzeros in CSR order for its k/W columns c = lane%W + u*W.  Each
-// column's sum is therefore the reference's simple_csr_matvec sum
-// (simple_blas.c:70-84) bit for bit, whatever W is.  W is picked per
-// (matrix, k) by timing the candidates once.
+// SpMM: one column per lane; a wave holds G = 64/k row groups of k lanes.
+// Each group takes RU rows per step and, for each row, issues the loads of
+// EU nonzeros (column index, value, the P entries) before summing them, so a
+// lane keeps RU*EU gathers in flight; the sums themselves run in CSR order,
+// one per column, which is the reference's simple_csr_matvec sum
+// (simple_blas.c:24-44) bit for bit whatever RU is.  RU (1, 2, 4) is picked
+// per (matrix, k) by timing the candidates once.
 //
 // Per-column scalars (rho, alpha, beta, tolrel, flags) live in a device
 // control block.  One iteration is five launches, graph-replayed:
@@ -39,7 +40,6 @@ namespace lap {
 namespace {
 
 constexpr int kMaxK = 64;         // columns per block solve
-constexpr int kMaxChunks = 8;     // columns per lane (k / W)
 constexpr int kRedThreads = 1024; // one-workgroup column reductions
 
 struct BlkCtl {
@@ -52,69 +52,91 @@ struct BlkCtl {
 
 // ---------------------------------------------------------------- SpMM
 
-// Q = A P for k columns.  MODE 0: plain (lap_csr_matmat); 1: partials of
+// Q = A P for K columns.  MODE 0: plain (lap_csr_matmat); 1: partials of
 // P.Q per column; 2: Q = -(A P - B) (initial residual) with partials of
-// Q.Q.  Partials: part[blockIdx.x * k + col].
-template <int BS, int W, int MODE>
-__global__ __launch_bounds__(BS) void k_spmm(int n, int k, const int *__restrict__ ia,
+// Q.Q.  Partials: part[blockIdx.x * K + col].
+template <int BS, int K, int RU, int EU, int MODE>
+__global__ __launch_bounds__(BS) void k_spmm(int n, const int *__restrict__ ia,
                                              const int *__restrict__ ja,
                                              const double *__restrict__ a,
                                              const double *__restrict__ P, double *__restrict__ Q,
                                              const double *__restrict__ Bm, double *part,
                                              const BlkCtl *ctl) {
   if (MODE == 1 && ctl->all_done) return;
-  constexpr int G = 64 / W, NW = BS / 64;
-  __shared__ double lds[kMaxK * NW];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-  const int g = lane / W, c = lane % W;
-  const int chunks = k / W;
-  double acc[kMaxChunks];
+  constexpr int G = 64 / K, NW = BS / 64;
+  __shared__ double lds[K * NW];
+  const int lane = threadIdx.x & 63;
+  // wave-uniform: with K = 64 every row index below is uniform, so the row
+  // pointers, column indices and values are scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int g = lane / K, c = lane % K;
+  const int wave_id = (int)blockIdx.x * NW + wave, nwaves = (int)gridDim.x * NW;
+  double acc = 0.0;
+  for (long base = (long)wave_id * G * RU; base < n; base += (long)nwaves * G * RU) {
+    int e0[RU], e1[RU];
 #pragma unroll
-  for (int u = 0; u < kMaxChunks; ++u) acc[u] = 0.0;
-  const int wave_id = (int)(blockIdx.x * NW + wave);
-  const int nwaves = (int)(gridDim.x * NW);
-  for (int row = wave_id * G + g; row < n; row += nwaves * G) {
-    const int e0 = ia[row], e1 = ia[row + 1];
-    double s[kMaxChunks];
-#pragma unroll
-    for (int u = 0; u < kMaxChunks; ++u) s[u] = 0.0;
-    for (int e = e0; e < e1; ++e) {
-      const double av = a[e];
-      const double *pr = P + (size_t)ja[e] * k + c;
-#pragma unroll
-      for (int u = 0; u < kMaxChunks; ++u)
-        if (u < chunks) s[u] += av * pr[u * W];
+    for (int j = 0; j < RU; ++j) {
+      const long row = base + (long)j * G + g;
+      e0[j] = row < n ? ia[row] : 0;
+      e1[j] = row < n ? ia[row + 1] : 0;
     }
-    const size_t o = (size_t)row * k + c;
+    double s[RU];
 #pragma unroll
-    for (int u = 0; u < kMaxChunks; ++u) {
-      if (u < chunks) {
+    for (int j = 0; j < RU; ++j) s[j] = 0.0;
+    for (int off = 0;; off += EU) {
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < RU; ++j) any |= e0[j] + off < e1[j];
+      if (!any) break;
+      bool ok[RU][EU];
+      int col[RU][EU];
+      double av[RU][EU], v[RU][EU];
+#pragma unroll
+      for (int j = 0; j < RU; ++j)
+#pragma unroll
+        for (int t = 0; t < EU; ++t) {
+          const int e = e0[j] + off + t;
+          ok[j][t] = e < e1[j];
+          col[j][t] = ok[j][t] ? ja[e] : 0;
+          av[j][t] = ok[j][t] ? a[e] : 0.0;
+        }
+#pragma unroll
+      for (int j = 0; j < RU; ++j)
+#pragma unroll
+        for (int t = 0; t < EU; ++t)
+          v[j][t] = ok[j][t] ? P[(size_t)col[j][t] * K + c] : 0.0;
+#pragma unroll
+      for (int j = 0; j < RU; ++j)
+#pragma unroll
+        for (int t = 0; t < EU; ++t)
+          if (ok[j][t]) s[j] += av[j][t] * v[j][t];
+    }
+#pragma unroll
+    for (int j = 0; j < RU; ++j) {
+      const long row = base + (long)j * G + g;
+      if (row < n) {
+        const size_t o = (size_t)row * K + c;
         if (MODE == 2) {
-          const double v = -(s[u] - Bm[o + u * W]);  // r = A x; r += -1*b; r *= -1
-          Q[o + u * W] = v;
-          acc[u] += v * v;
+          const double r = -(s[j] - Bm[o]);  // r = A x; r += -1*b; r *= -1
+          Q[o] = r;
+          acc += r * r;
         } else {
-          Q[o + u * W] = s[u];
-          if (MODE == 1) acc[u] += P[o + u * W] * s[u];
+          Q[o] = s[j];
+          if (MODE == 1) acc += P[o] * s[j];
         }
       }
     }
   }
   if (MODE == 0) return;
-  // fixed-order reduction: butterfly over the lane groups of a wave, then
+  // fixed-order reduction: butterfly over the row groups of a wave, then
   // over the waves of the workgroup
-#pragma unroll
-  for (int u = 0; u < kMaxChunks; ++u) {
-    if (u < chunks) {
-      for (int off = W; off < 64; off <<= 1) acc[u] += __shfl_xor(acc[u], off);
-      if (g == 0) lds[(u * W + c) * NW + wave] = acc[u];
-    }
-  }
+  for (int off = K; off < 64; off <<= 1) acc += __shfl_xor(acc, off);
+  if (g == 0) lds[c * NW + wave] = acc;
   __syncthreads();
-  for (int col = threadIdx.x; col < k; col += BS) {
+  for (int cc = threadIdx.x; cc < K; cc += BS) {
     double t = 0.0;
-    for (int w = 0; w < NW; ++w) t += lds[col * NW + w];
-    part[(size_t)blockIdx.x * k + col] = t;
+    for (int w = 0; w < NW; ++w) t += lds[cc * NW + w];
+    part[(size_t)blockIdx.x * K + cc] = t;
   }
 }
 
@@ -311,37 +333,46 @@ __global__ __launch_bounds__(BS) void k_blk_dir(int n, int k, int lk, double *__
   }
 }
 
-using SpmmFn = void (*)(unsigned, int, int, const int *, const int *, const double *,
-                        const double *, double *, const double *, double *, const BlkCtl *, int);
+using SpmmFn = void (*)(unsigned, int, const int *, const int *, const double *, const double *,
+                        double *, const double *, double *, const BlkCtl *, int);
 
-template <int W>
-void spmm_launch(unsigned g, int n, int k, const int *ia, const int *ja, const double *a,
+template <int K, int RU>
+void spmm_launch(unsigned g, int n, const int *ia, const int *ja, const double *a,
                  const double *P, double *Q, const double *Bm, double *part, const BlkCtl *ctl,
                  int mode) {
+  constexpr int EU = RU == 1 ? 8 : 4;
   if (mode == 2)
-    k_spmm<kBlock, W, 2><<<g, kBlock, 0, stream()>>>(n, k, ia, ja, a, P, Q, Bm, part, ctl);
+    k_spmm<kBlock, K, RU, EU, 2><<<g, kBlock, 0, stream()>>>(n, ia, ja, a, P, Q, Bm, part, ctl);
   else if (mode == 1)
-    k_spmm<kBlock, W, 1><<<g, kBlock, 0, stream()>>>(n, k, ia, ja, a, P, Q, Bm, part, ctl);
+    k_spmm<kBlock, K, RU, EU, 1><<<g, kBlock, 0, stream()>>>(n, ia, ja, a, P, Q, Bm, part, ctl);
   else
-    k_spmm<kBlock, W, 0><<<g, kBlock, 0, stream()>>>(n, k, ia, ja, a, P, Q, Bm, part, ctl);
+    k_spmm<kBlock, K, RU, EU, 0><<<g, kBlock, 0, stream()>>>(n, ia, ja, a, P, Q, Bm, part, ctl);
 }
 
-SpmmFn spmm_for(int W) {
-  switch (W) {
-    case 1: return spmm_launch<1>;
-    case 2: return spmm_launch<2>;
-    case 4: return spmm_launch<4>;
-    case 8: return spmm_launch<8>;
-    case 16: return spmm_launch<16>;
-    case 32: return spmm_launch<32>;
-    case 64: return spmm_launch<64>;
+template <int K>
+SpmmFn spmm_for_k(int ru) {
+  switch (ru) {
+    case 1: return spmm_launch<K, 1>;
+    case 2: return spmm_launch<K, 2>;
+    case 4: return spmm_launch<K, 4>;
     default: return nullptr;
   }
 }
 
-bool tile_ok(int k, int W) {
-  return W >= 1 && W <= 64 && (64 % W) == 0 && (k % W) == 0 && k / W <= kMaxChunks;
+SpmmFn spmm_for(int k, int ru) {
+  switch (k) {
+    case 1: return spmm_for_k<1>(ru);
+    case 2: return spmm_for_k<2>(ru);
+    case 4: return spmm_for_k<4>(ru);
+    case 8: return spmm_for_k<8>(ru);
+    case 16: return spmm_for_k<16>(ru);
+    case 32: return spmm_for_k<32>(ru);
+    case 64: return spmm_for_k<64>(ru);
+    default: return nullptr;
+  }
 }
+
+bool tile_ok(int ru) { return ru == 1 || ru == 2 || ru == 4; }
 
 int log2i(int k) {
   int l = 0;
@@ -359,13 +390,14 @@ unsigned vec_grid(size_t elems) {
       1, std::min<size_t>(1024, (elems + 4 * kBlock - 1) / (4 * kBlock)));
 }
 
-// Times every admissible W on Y = A X (three runs each, after a warm-up).
+// Times every rows-per-group choice on Y = A X (three runs each, after a
+// warm-up).
 int pick_tile(int n, const int *ia, const int *ja, const double *a, int k, const double *X,
               double *Y) {
   auto key = std::make_tuple((const void *)ia, (const void *)ja, (const void *)a, n, k);
   auto it = g_tile.find(key);
   if (it != g_tile.end()) return it->second;
-  int best_w = 0;
+  int best_ru = 0;
   float best = 1e30f;
   const unsigned g = spmm_grid();
   hipEvent_t e0, e1;
@@ -374,25 +406,24 @@ int pick_tile(int n, const int *ia, const int *ja, const double *a, int k, const
     (void)hipEventDestroy(e0);
     return 0;
   }
-  for (int W = 1; W <= 64; W *= 2) {
-    if (!tile_ok(k, W)) continue;
-    SpmmFn fn = spmm_for(W);
-    fn(g, n, k, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
+  for (int ru = 1; ru <= 4; ru *= 2) {
+    SpmmFn fn = spmm_for(k, ru);
+    fn(g, n, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
     LAP_HIP(hipEventRecord(e0, stream()));
-    for (int r = 0; r < 3; ++r) fn(g, n, k, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
+    for (int r = 0; r < 3; ++r) fn(g, n, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
     LAP_HIP(hipEventRecord(e1, stream()));
     LAP_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     LAP_HIP(hipEventElapsedTime(&ms, e0, e1));
     if (ms < best) {
       best = ms;
-      best_w = W;
+      best_ru = ru;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (best_w) g_tile[key] = best_w;
-  return best_w;
+  if (best_ru) g_tile[key] = best_ru;
+  return best_ru;
 }
 
 bool block_args(const char *fn, int n, int nnz, const int *ia, const int *ja, const double *a,
@@ -403,9 +434,8 @@ bool block_args(const char *fn, int n, int nnz, const int *ia, const int *ja, co
               kMaxK, n, k);
     return false;
   }
-  if (tile && !tile_ok(k, tile)) {
-    set_error(LAP_ERR_ARG, "%s: tile %d must divide 64 and k=%d with k/tile <= %d", fn, tile, k,
-              kMaxChunks);
+  if (tile && !tile_ok(tile)) {
+    set_error(LAP_ERR_ARG, "%s: tile (rows per lane group) must be 1, 2 or 4, not %d", fn, tile);
     return false;
   }
   return check_device(fn, {{ia, "ia", sizeof(int) * ((size_t)n + 1)},
@@ -438,7 +468,7 @@ extern "C" int lap_csr_matmat(int n, int nnz, const int *ia, const int *ja, cons
   if (!spmv_plan(n, nnz, ia, ja)) return lap_last_status();  // validates ia / ja
   if (!tile) tile = pick_tile(n, ia, ja, a, k, X, Y);
   if (!tile) return LAP_ERR_HIP;
-  spmm_for(tile)(spmm_grid(), n, k, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
+  spmm_for(k, tile)(spmm_grid(), n, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
   if (!LAP_HIP(hipGetLastError())) return LAP_ERR_HIP;
   sync();
   return LAP_OK;
@@ -475,7 +505,7 @@ extern "C" int lap_pcg_block(int n, int nnz, const int *ia, const int *ja, const
   if (hist) launch_fill((int)hist_len, 0.0, hist);
   if (!tile) tile = pick_tile(n, ia, ja, a, k, Bm, Q);
   if (!tile) return LAP_ERR_HIP;
-  SpmmFn spmm = spmm_for(tile);
+  SpmmFn spmm = spmm_for(k, tile);
   const unsigned g = spmm_grid(), gv = vec_grid((size_t)n * k);
   const int lk = log2i(k);
 
@@ -486,12 +516,12 @@ extern "C" int lap_pcg_block(int n, int nnz, const int *ia, const int *ja, const
     return LAP_ERR_HIP;
   }
   LAP_HIP(hipEventRecord(ev0, stream()));
-  spmm(g, n, k, ia, ja, a, X, R, Bm, part, ctl, 2);
+  spmm(g, n, ia, ja, a, X, R, Bm, part, ctl, 2);
   k_blk_res0<<<1, kRedThreads, 0, stream()>>>(part, (int)g, k, tol, maxit, ctl, hist);
   k_blk_start<kBlock><<<gv, kBlock, 0, stream()>>>(n, k, lk, R, d_r, P, part);
   k_blk_rho<<<1, kRedThreads, 0, stream()>>>(part, (int)gv, k, ctl);
   auto iteration = [&]() {
-    spmm(g, n, k, ia, ja, a, P, Q, nullptr, part, ctl, 1);
+    spmm(g, n, ia, ja, a, P, Q, nullptr, part, ctl, 1);
     k_blk_alpha<<<1, kRedThreads, 0, stream()>>>(part, (int)g, k, ctl);
     k_blk_update<kBlock><<<gv, kBlock, 0, stream()>>>(n, k, lk, R, Q, d_r, ctl, part);
     k_blk_conv<<<1, kRedThreads, 0, stream()>>>(part, (int)gv, k, hist, ctl);

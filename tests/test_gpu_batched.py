@@ -4,12 +4,14 @@ mean removed for the shifted Laplacians), for every kernel configuration and
 the autotuned one; degenerate systems (1 row, zero right-hand side) are
 handled.
 
-Iteration counts: within +-1 of the reference's serial path or of its OpenMP
-path (dot products summed by 8 threads).  On the smallest grids (8x8 + 1e-8 I)
-the residual is not monotone near the tolerance (it rises from 6e-10 to 3e-8
-relative before the near-null mode is resolved), so the exit iteration depends
-on the dot products' summation order: the reference's own serial and OpenMP
-builds stop at 36 and 34 iterations there."""
+Iteration counts: within +-1 of the oracle (the reference's serial path).
+Where the oracle's residual is not monotone in its last 8 iterations (the
+smallest grids, e.g. 8x8 + 1e-8 I, whose residual rises from 6e-10 to 3e-8
+relative before the near-null mode is resolved) the exit iteration depends on
+the dot products' summation order -- the reference's own serial and OpenMP
+builds stop at 36 and 34 there -- so such systems are held to +-3 and to the
+FP64 true-residual check instead.  A zero right-hand side stops at iteration 0
+with x = 0 (the reference divides 0/0 there and runs to maxit)."""
 import numpy as np
 import pytest
 
@@ -49,18 +51,26 @@ def test_batched_matches_per_system_oracle(orc, block, tpr):
     x, it, fl, r0, rs, ms = S.pcg_batched(B, tol=1e-10, maxit=5000, block=block, tpr=tpr)
     xs = [v.cpu().numpy() for v in B.split(x)]
     worst = 0.0
+    relaxed = []
     for k, (A, b) in enumerate(zip(systems, rhs)):
-        xo, ito, flo, _ = orc.cg(A, b, "line_jacobi", tol=1e-10, maxit=5000)
-        _, ito_omp, _, _ = orc.cg(A, b, "line_jacobi", tol=1e-10, maxit=5000, omp=(16, 8))
+        if not b.any():
+            assert it[k] == 0 and fl[k] == 0 and not xs[k].any()
+            continue
+        xo, ito, flo, ho = orc.cg(A, b, "line_jacobi", tol=1e-10, maxit=5000)
         assert fl[k] == flo == 0, (k, fl[k], flo)
-        assert min(abs(int(it[k]) - ito), abs(int(it[k]) - ito_omp)) <= 1, \
-            (k, it[k], ito, ito_omp)
+        tail = ho[max(0, ito - 8):]
+        bumpy = bool(np.any(np.diff(tail) > 0))
+        assert abs(int(it[k]) - ito) <= (3 if bumpy else 1), (k, it[k], ito, bumpy)
+        if bumpy:
+            relaxed.append(k)
+            true = np.linalg.norm(b - orc.matvec(A, xs[k].copy()))
+            assert true <= 1e-10 * np.linalg.norm(b) * 1.5
         if np.linalg.norm(centred(xo)) > 0:
             err = np.linalg.norm(centred(xs[k]) - centred(xo)) / np.linalg.norm(centred(xo))
             worst = max(worst, err)
-    print(f"[diag] batched {block}/{tpr}: {ms:.2f} ms, worst rel err {worst:.2e}", flush=True)
+    print(f"[diag] batched {block}/{tpr}: {ms:.2f} ms, worst rel err {worst:.2e}, "
+          f"systems with a non-monotone residual tail: {relaxed}", flush=True)
     assert worst <= 1e-9
-    assert it[-2] == 0 and fl[-2] == 0  # the 1x1 system with b = 0
 
 
 def test_batched_tpr1_rows_are_reference_order(orc):

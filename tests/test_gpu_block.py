@@ -2,7 +2,7 @@
 
 Bars:
   * SpMM: every column bit-identical to the oracle's CSR matvec
-    (simple_csr_matvec order), for every admissible lane-group width;
+    (simple_csr_matvec order), for every rows-per-group setting;
   * fixed iteration count (maxit-bounded): every column's iterate and
     residual history match the oracle's single-RHS line_jacobi PCG to 1e-9
     relative (only the dot products' summation order differs);
@@ -51,11 +51,9 @@ def test_spmm_columns_bitwise(orc, pname, k):
     X = np.random.default_rng(k).standard_normal((Ah.n, k))
     Xd = S.to_dev(X)
     want = np.stack([orc.matvec(Ah, X[:, j].copy()) for j in range(k)], axis=1)
-    for W in (1, 2, 4, 8, 16, 32, 64):
-        if k % W or k // W > 8:
-            continue
-        Y = S.spmm(A, Xd, tile=W).cpu().numpy()
-        assert np.array_equal(Y, want), (W, np.abs(Y - want).max())
+    for ru in (1, 2, 4):  # rows per lane group
+        Y = S.spmm(A, Xd, tile=ru).cpu().numpy()
+        assert np.array_equal(Y, want), (ru, np.abs(Y - want).max())
     Y = S.spmm(A, Xd).cpu().numpy()  # timed choice
     assert np.array_equal(Y, want)
 
@@ -76,7 +74,7 @@ def test_block_fixed_iterations_vs_oracle(orc, pname):
         assert rel(X[:, j], xo) <= 1e-9, j
 
 
-@pytest.mark.parametrize("tile", [0, 8, 64])
+@pytest.mark.parametrize("tile", [0, 1, 4])
 def test_block_converged_vs_oracle(orc, tile):
     """Columns of different difficulty stop at different iterations."""
     print(f"[case] block converged tile={tile}", flush=True)
@@ -155,6 +153,6 @@ def test_block_rejects_bad_arguments():
     with pytest.raises(L.LapError):
         S.pcg_block(A, S.to_dev(rhs_block(Ah.n, 3, 1)), d_r)  # k not a power of two
     with pytest.raises(L.LapError):
-        S.pcg_block(A, S.to_dev(rhs_block(Ah.n, 4, 1)), d_r, tile=8)  # tile does not divide k
+        S.pcg_block(A, S.to_dev(rhs_block(Ah.n, 4, 1)), d_r, tile=3)  # not 1, 2 or 4
     with pytest.raises(L.LapError):
         S.spmm(A, S.to_dev(rhs_block(Ah.n, 128, 1)))  # k > 64
