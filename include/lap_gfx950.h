@@ -241,7 +241,11 @@ int lap_pcg_mixed(int n, int nnz, const int *ia, const int *ja, const double *a,
  * systems from an atomic queue (largest first) and run each system's PCG
  * (cg.c recurrence, line_jacobi) with workgroup-level reductions only.
  * block (256/512/1024) and tpr (threads per row: 1/2/4) select the kernel;
- * 0/0 uses the configuration found by lap_pcg_batched_tune (or 256/1).
+ * block = -1 selects the lockstep mode instead (every system iterates
+ * together over the block-diagonal matrix, several launches per iteration
+ * from a HIP graph, stopped systems skipped: for batches whose largest
+ * system would dominate one workgroup); 0/0 uses the configuration found by
+ * lap_pcg_batched_tune (or 256/1).
  * Per-system outputs (host arrays of nsys, each may be NULL): iterations,
  * flag (0/1/2 as cg()), ||r0|| and the final recursive residual. */
 int lap_pcg_batched(int nsys, const int *row_off, int n_total, int nnz_total, const int *ia,
@@ -249,8 +253,9 @@ int lap_pcg_batched(int nsys, const int *row_off, int n_total, int nnz_total, co
                     const double *b, double tol, int maxit, int block, int tpr, int *iters,
                     int *flags, double *res0, double *res, double *gpu_ms);
 /* Autotuner for lap_pcg_batched: times trial_iters iterations of every
- * (block, tpr) kernel on this batch (x = 0, the caller's x is not touched)
- * and keeps the fastest for later calls with block = 0. */
+ * (block, tpr) kernel and of the lockstep mode on this batch (x = 0, the
+ * caller's x is not touched) and keeps the fastest for later calls with
+ * block = 0; *best_block = -1 when the lockstep mode won. */
 int lap_pcg_batched_tune(int nsys, const int *row_off, int n_total, int nnz_total,
                          const int *ia, const int *ja, const double *a, const double *d_r,
                          const double *b, int trial_iters, int *best_block, int *best_tpr);

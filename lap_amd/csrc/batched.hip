@@ -13,11 +13,22 @@
 // by the queue.  Threads per row in the SpMV and the workgroup size are
 // chosen per batch by a measured autotuner (lap_pcg_batched_tune).  With one
 // thread per row every row is summed in CSR order, as the reference does.
+//
+// Lockstep mode (block = -1): a batch whose largest system dominates is
+// bounded by that system's time on one CU in the persistent kernel.  In
+// lockstep every system iterates together over the block-diagonal matrix:
+// row blocks never cross a system boundary, each iteration is five launches
+// (SpMV with per-block p.q partials; one workgroup computing alpha per
+// system; x/r update with per-block partials; one workgroup doing every
+// system's convergence test and beta; p update) replayed from a HIP graph,
+// and the row blocks of systems that have stopped are skipped.  The
+// autotuner times it next to the persistent configurations.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
 
 #include "lap_core.hpp"
+#include "spmv_core.hpp"
 
 namespace lap {
 
@@ -147,6 +158,282 @@ __global__ __launch_bounds__(BS) void k_batched(
   }
 }
 
+
+// ------------------------------------------------------------ lockstep
+
+// Per-system state of the lockstep mode (arrays of nsys).
+struct LsSys {
+  double *rho, *alpha, *beta, *tolrel, *res, *res0;
+  int *it, *flag, *done;
+};
+struct LsCtl {
+  int iter, maxit, all_done, pad;
+};
+
+constexpr int kLsRed = 1024;  // one-workgroup per-system reductions
+
+// Sum of part[b*stride] over blocks [b0, b1) in a fixed order, returned to
+// every lane of the wave.
+__device__ __forceinline__ double ls_sys_sum(const double *part, int stride, int b0, int b1) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int b = b0 + lane; b < b1; b += 64) s += part[(size_t)b * stride];
+  return wave_sum(s);
+}
+
+struct LsInitEpi {
+  double *r;
+  const double *rhs;
+  double acc;
+  __device__ double init(int) const { return 0.0; }
+  __device__ void store(int row, double v) {
+    const double t = -(v - rhs[row]);  // r = A x; r += -1*b; r *= -1
+    r[row] = t;
+    acc += t * t;
+  }
+};
+
+struct LsPqEpi {
+  double *q;
+  const double *p;
+  double acc;
+  __device__ double init(int) const { return 0.0; }
+  __device__ void store(int row, double v) {
+    q[row] = v;
+    acc += p[row] * v;
+  }
+};
+
+// r = -(A x - b), per-block partials of r.r
+template <int BS, int TILE>
+__global__ __launch_bounds__(BS) void k_ls_init(int nblk, const int *__restrict__ rowblk,
+                                                const int *__restrict__ ia,
+                                                const int *__restrict__ ja,
+                                                const double *__restrict__ a,
+                                                const double *__restrict__ x,
+                                                const double *__restrict__ b,
+                                                double *__restrict__ r, double *part) {
+  __shared__ double prod[TILE];
+  __shared__ double red[BS / kWave];
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    LsInitEpi epi{r, b, 0.0};
+    spmv_block_rows<BS, TILE>(blk, rowblk, ia, ja, a, x, 1.0, prod, red, epi);
+    const double t = block_sum<BS>(epi.acc, red);
+    if (threadIdx.x == 0) part[blk] = t;
+  }
+}
+
+// ||r0|| per system, tolrel, stop state; ctl
+__global__ __launch_bounds__(kLsRed) void k_ls_start(int nsys, const int *__restrict__ sblk,
+                                                     const double *part, double tol, int maxit,
+                                                     LsSys S, LsCtl *ctl) {
+  __shared__ int live[kLsRed / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int cnt = 0;
+  for (int s = wave; s < nsys; s += kLsRed / 64) {
+    const double r0 = sqrt(ls_sys_sum(part, 1, sblk[s], sblk[s + 1]));
+    if (lane == 0) {
+      const bool stop = !isfinite(r0) || r0 == 0.0;  // r0 == 0: x0 solves system s
+      S.res0[s] = r0;
+      S.res[s] = r0;
+      S.tolrel[s] = tol * r0;
+      S.it[s] = 0;
+      S.flag[s] = isfinite(r0) ? 0 : 2;
+      S.done[s] = stop ? 1 : 0;
+      cnt += stop ? 0 : 1;
+    }
+  }
+  if (lane == 0) live[wave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kLsRed / 64; ++w) t += live[w];
+    ctl->iter = 0;
+    ctl->maxit = maxit;
+    ctl->all_done = t ? 0 : 1;
+  }
+}
+
+// Block-uniform "system of this block has stopped" test.
+__device__ __forceinline__ bool ls_skip(int blk, const int *__restrict__ bsys,
+                                        const int *__restrict__ done) {
+  return __builtin_amdgcn_readfirstlane(done[bsys[blk]]) != 0;
+}
+
+// w = d_r r, p = w (first iteration), per-block partials of r.w
+template <int BS>
+__global__ __launch_bounds__(BS) void k_ls_jstart(int nblk, const int *__restrict__ rowblk,
+                                                  const int *__restrict__ bsys, const int *done,
+                                                  const double *__restrict__ r,
+                                                  const double *__restrict__ d_r,
+                                                  double *__restrict__ p, double *part) {
+  __shared__ double red[BS / kWave];
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    if (ls_skip(blk, bsys, done)) continue;
+    const int r0 = rowblk[blk], r1 = rowblk[blk + 1];
+    double acc = 0.0;
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += BS) {
+      const double ri = r[i];
+      const double wi = d_r[i] * ri;  // line_jacobi: w = d_r .* r
+      p[i] = wi;
+      acc += ri * wi;
+    }
+    const double t = block_sum<BS>(acc, red);
+    if (threadIdx.x == 0) part[blk] = t;
+  }
+}
+
+// rho per live system
+__global__ __launch_bounds__(kLsRed) void k_ls_rho(int nsys, const int *__restrict__ sblk,
+                                                   const double *part, LsSys S) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int s = wave; s < nsys; s += kLsRed / 64) {
+    if (S.done[s]) continue;
+    const double rho = ls_sys_sum(part, 1, sblk[s], sblk[s + 1]);
+    if (lane == 0) S.rho[s] = rho;
+  }
+}
+
+// q = A p for the live systems' blocks, per-block partials of p.q
+template <int BS, int TILE>
+__global__ __launch_bounds__(BS) void k_ls_spmv(int nblk, const int *__restrict__ rowblk,
+                                                const int *__restrict__ bsys, const int *done,
+                                                const int *__restrict__ ia,
+                                                const int *__restrict__ ja,
+                                                const double *__restrict__ a,
+                                                const double *__restrict__ p,
+                                                double *__restrict__ q, double *part,
+                                                const LsCtl *ctl) {
+  if (__builtin_amdgcn_readfirstlane(ctl->all_done)) return;
+  __shared__ double prod[TILE];
+  __shared__ double red[BS / kWave];
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    if (ls_skip(blk, bsys, done)) continue;
+    LsPqEpi epi{q, p, 0.0};
+    spmv_block_rows<BS, TILE>(blk, rowblk, ia, ja, a, p, 1.0, prod, red, epi);
+    const double t = block_sum<BS>(epi.acc, red);
+    if (threadIdx.x == 0) part[blk] = t;
+  }
+}
+
+// alpha = rho / p.q per live system
+__global__ __launch_bounds__(kLsRed) void k_ls_alpha(int nsys, const int *__restrict__ sblk,
+                                                     const double *part, LsSys S,
+                                                     const LsCtl *ctl) {
+  if (ctl->all_done) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int s = wave; s < nsys; s += kLsRed / 64) {
+    if (S.done[s]) continue;
+    const double pq = ls_sys_sum(part, 1, sblk[s], sblk[s + 1]);
+    if (lane == 0) S.alpha[s] = S.rho[s] / pq;
+  }
+}
+
+// x += alpha p; r -= alpha q; per-block partials of r.w and r.r
+template <int BS>
+__global__ __launch_bounds__(BS) void k_ls_update(int nblk, const int *__restrict__ rowblk,
+                                                  const int *__restrict__ bsys, LsSys S,
+                                                  double *__restrict__ x,
+                                                  double *__restrict__ r,
+                                                  const double *__restrict__ p,
+                                                  const double *__restrict__ q,
+                                                  const double *__restrict__ d_r, double *part,
+                                                  const LsCtl *ctl) {
+  if (__builtin_amdgcn_readfirstlane(ctl->all_done)) return;
+  __shared__ double red[BS / kWave];
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    if (ls_skip(blk, bsys, S.done)) continue;
+    const int sys = bsys[blk];
+    const double alpha = S.alpha[sys], nalpha = -1.0 * alpha;
+    const int r0 = rowblk[blk], r1 = rowblk[blk + 1];
+    double rz = 0.0, rr = 0.0;
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += BS) {
+      x[i] += alpha * p[i];
+      const double ri = r[i] + nalpha * q[i];
+      r[i] = ri;
+      rz += ri * (d_r[i] * ri);
+      rr += ri * ri;
+    }
+    const double t_rz = block_sum<BS>(rz, red);
+    const double t_rr = block_sum<BS>(rr, red);
+    if (threadIdx.x == 0) {
+      part[2 * (size_t)blk] = t_rz;
+      part[2 * (size_t)blk + 1] = t_rr;
+    }
+  }
+}
+
+// per-system convergence test (cg.c:106-123) and beta; all_done
+__global__ __launch_bounds__(kLsRed) void k_ls_conv(int nsys, const int *__restrict__ sblk,
+                                                    const double *part, LsSys S, LsCtl *ctl) {
+  if (ctl->all_done) return;
+  __shared__ int live[kLsRed / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int maxit = ctl->maxit;
+  int cnt = 0;
+  for (int s = wave; s < nsys; s += kLsRed / 64) {
+    if (S.done[s]) continue;
+    const double rz = ls_sys_sum(part, 2, sblk[s], sblk[s + 1]);
+    const double rr = ls_sys_sum(part + 1, 2, sblk[s], sblk[s + 1]);
+    if (lane == 0) {
+      const int it = S.it[s] + 1;
+      const double res = sqrt(rr);
+      S.it[s] = it;
+      S.res[s] = res;
+      int flag = -1;
+      if (!isfinite(res))
+        flag = 2;
+      else if (res < S.tolrel[s])
+        flag = 0;
+      else if (it > maxit)
+        flag = 1;
+      if (flag >= 0) {
+        S.flag[s] = flag;
+        S.done[s] = 1;
+      } else {
+        S.beta[s] = rz / S.rho[s];
+        S.rho[s] = rz;
+        ++cnt;
+      }
+    }
+  }
+  if (lane == 0) live[wave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kLsRed / 64; ++w) t += live[w];
+    ctl->iter += 1;
+    ctl->all_done = t ? 0 : 1;
+  }
+}
+
+// p = beta p + d_r r for the live systems
+template <int BS>
+__global__ __launch_bounds__(BS) void k_ls_p(int nblk, const int *__restrict__ rowblk,
+                                             const int *__restrict__ bsys, LsSys S,
+                                             const double *__restrict__ r,
+                                             const double *__restrict__ d_r,
+                                             double *__restrict__ p, const LsCtl *ctl) {
+  if (__builtin_amdgcn_readfirstlane(ctl->all_done)) return;
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    if (ls_skip(blk, bsys, S.done)) continue;
+    const double beta = S.beta[bsys[blk]];
+    const int r0 = rowblk[blk], r1 = rowblk[blk + 1];
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += BS) p[i] = p[i] * beta + d_r[i] * r[i];
+  }
+}
+
+// Row blocks per system: at most kLsRows rows and kSpmvTile nonzeros per
+// block (a longer row gets a block of its own), never crossing systems.
+constexpr int kLsRows = 256;
+
+struct LsPlan {
+  int nblk = 0;
+  DevBufPtr rowblk, bsys, sblk, sys;  // sys: LsSys arrays + LsCtl
+  LsSys S{};
+  LsCtl *ctl = nullptr;
+};
+
 struct BatchCfg {
   int bs = 256, tpr = 1;
 };
@@ -187,6 +474,8 @@ struct Batch {
   int nsys = 0, n = 0;
   DevBufPtr order, it_out, flag_out, res0_out, res_out, queue;
   int tuned_bs = 0, tuned_tpr = 0;
+  std::unique_ptr<LsPlan> ls;
+  std::vector<int> off;  // host copy of row_off
 };
 using BKey = std::tuple<const int *, int, int>;
 std::map<BKey, std::unique_ptr<Batch>> g_batches;
@@ -213,6 +502,7 @@ Batch *get_batch(int nsys, const int *row_off, int n_total) {
   auto B = std::make_unique<Batch>();
   B->nsys = nsys;
   B->n = n_total;
+  B->off = off;
   B->order = dev_upload(ord);
   B->it_out = dev_alloc(sizeof(int) * (size_t)nsys);
   B->flag_out = dev_alloc(sizeof(int) * (size_t)nsys);
@@ -248,6 +538,142 @@ float run_cfg(const CfgEntry &c, Batch &B, const int *row_off, const int *ia, co
   LAP_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  return ms;
+}
+
+
+// Lockstep plan of a batch (host analysis of the row pointers).
+LsPlan *ls_plan(Batch &B, const int *ia) {
+  if (B.ls) return B.ls.get();
+  std::vector<int> h_ia = dev_download(ia, (size_t)B.n + 1);
+  std::vector<int> rowblk, bsys, sblk;
+  rowblk.push_back(0);
+  sblk.push_back(0);
+  for (int s = 0; s < B.nsys; ++s) {
+    int r = B.off[s];
+    const int end = B.off[s + 1];
+    while (r < end) {
+      int r1 = r + 1;
+      if (h_ia[r1] - h_ia[r] <= kSpmvTile)
+        while (r1 < end && r1 - r < kLsRows && h_ia[r1 + 1] - h_ia[r] <= kSpmvTile) ++r1;
+      rowblk.push_back(r1);
+      bsys.push_back(s);
+      r = r1;
+    }
+    sblk.push_back((int)bsys.size());
+  }
+  auto P = std::make_unique<LsPlan>();
+  P->nblk = (int)bsys.size();
+  P->rowblk = dev_upload(rowblk);
+  P->bsys = dev_upload(bsys.empty() ? std::vector<int>{0} : bsys);
+  P->sblk = dev_upload(sblk);
+  const size_t ns = (size_t)B.nsys;
+  const size_t dbytes = sizeof(double) * ns * 6, ibytes = sizeof(int) * ns * 3;
+  P->sys = dev_alloc(dbytes + ibytes + sizeof(LsCtl) + 64);
+  if (!P->rowblk || !P->bsys || !P->sblk || !P->sys) return nullptr;
+  char *base = P->sys->as<char>();
+  double *d = reinterpret_cast<double *>(base);
+  P->S = LsSys{d, d + ns, d + 2 * ns, d + 3 * ns, d + 4 * ns, d + 5 * ns, nullptr, nullptr,
+               nullptr};
+  int *iv = reinterpret_cast<int *>(base + dbytes);
+  P->S.it = iv;
+  P->S.flag = iv + ns;
+  P->S.done = iv + 2 * ns;
+  P->ctl = reinterpret_cast<LsCtl *>(base + ((dbytes + ibytes + 15) & ~(size_t)15));
+  B.ls = std::move(P);
+  return B.ls.get();
+}
+
+// One lockstep solve; outputs land in B's output arrays.  Returns the
+// device milliseconds (HIP events), or -1 on error.
+float run_lockstep(Batch &B, const int *ia, const int *ja, const double *a, const double *d_r,
+                   double *x, const double *b, double tol, int maxit) {
+  LsPlan *P = ls_plan(B, ia);
+  if (!P) return -1.f;
+  const int nblk = P->nblk;
+  double *r = scratch_vec("bt_r", (size_t)B.n), *p = scratch_vec("bt_p", (size_t)B.n),
+         *q = scratch_vec("bt_q", (size_t)B.n);
+  double *part = scratch_vec("ls_part", (size_t)5 * std::max(nblk, 1));
+  if (!r || !p || !q || !part) return -1.f;
+  double *part0 = part, *part1 = part + nblk, *part2 = part + 2 * (size_t)nblk,
+         *part3 = part + 3 * (size_t)nblk;
+  const int *rowblk = P->rowblk->as<int>(), *bsys = P->bsys->as<int>(),
+            *sblk = P->sblk->as<int>();
+  const unsigned g = (unsigned)std::max(1, std::min(nblk, 8 * device_cus()));
+  hipEvent_t e0, e1;
+  if (!LAP_HIP(hipEventCreate(&e0))) return -1.f;
+  if (!LAP_HIP(hipEventCreate(&e1))) {
+    (void)hipEventDestroy(e0);
+    return -1.f;
+  }
+  LAP_HIP(hipEventRecord(e0, stream()));
+  k_ls_init<kBlock, kSpmvTile><<<g, kBlock, 0, stream()>>>(nblk, rowblk, ia, ja, a, x, b, r,
+                                                          part0);
+  k_ls_start<<<1, kLsRed, 0, stream()>>>(B.nsys, sblk, part0, tol, maxit, P->S, P->ctl);
+  k_ls_jstart<kBlock><<<g, kBlock, 0, stream()>>>(nblk, rowblk, bsys, P->S.done, r, d_r, p,
+                                                  part1);
+  k_ls_rho<<<1, kLsRed, 0, stream()>>>(B.nsys, sblk, part1, P->S);
+  auto iteration = [&]() {
+    k_ls_spmv<kBlock, kSpmvTile><<<g, kBlock, 0, stream()>>>(nblk, rowblk, bsys, P->S.done, ia,
+                                                            ja, a, p, q, part2, P->ctl);
+    k_ls_alpha<<<1, kLsRed, 0, stream()>>>(B.nsys, sblk, part2, P->S, P->ctl);
+    k_ls_update<kBlock><<<g, kBlock, 0, stream()>>>(nblk, rowblk, bsys, P->S, x, r, p, q, d_r,
+                                                    part3, P->ctl);
+    k_ls_conv<<<1, kLsRed, 0, stream()>>>(B.nsys, sblk, part3, P->S, P->ctl);
+    k_ls_p<kBlock><<<g, kBlock, 0, stream()>>>(nblk, rowblk, bsys, P->S, r, d_r, p, P->ctl);
+  };
+  // every system stops by iteration maxit + 1, so G * max_batches
+  // iterations always reach all_done
+  const int G = std::max(1, std::min(options().graph_iters, maxit + 2));
+  const long max_batches = (long)(maxit + 1) / G + 3;
+  bool ok = true;
+  hipGraphExec_t gexec = nullptr;
+  if (options().use_graph) {
+    hipGraph_t graph = nullptr;
+    ok = LAP_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    if (ok) {
+      for (int i = 0; i < G; ++i) iteration();
+      ok = LAP_HIP(hipStreamEndCapture(stream(), &graph));
+    }
+    if (ok) ok = LAP_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  int *flag_h = pinned_flags();
+  bool finished = false;
+  if (ok && flag_h) {
+    for (long it = 0; it < max_batches && !finished; ++it) {
+      if (gexec) {
+        if (!LAP_HIP(hipGraphLaunch(gexec, stream()))) break;
+      } else {
+        for (int i = 0; i < G; ++i) iteration();
+      }
+      LAP_HIP(hipMemcpyAsync(flag_h, &P->ctl->all_done, sizeof(int), hipMemcpyDeviceToHost,
+                             stream()));
+      sync();
+      finished = flag_h[0] != 0;
+    }
+  }
+  LAP_HIP(hipEventRecord(e1, stream()));
+  LAP_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  LAP_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  const size_t ns = (size_t)B.nsys;
+  LAP_HIP(hipMemcpyAsync(B.it_out->p, P->S.it, sizeof(int) * ns, hipMemcpyDeviceToDevice,
+                         stream()));
+  LAP_HIP(hipMemcpyAsync(B.flag_out->p, P->S.flag, sizeof(int) * ns, hipMemcpyDeviceToDevice,
+                         stream()));
+  LAP_HIP(hipMemcpyAsync(B.res0_out->p, P->S.res0, sizeof(double) * ns,
+                         hipMemcpyDeviceToDevice, stream()));
+  LAP_HIP(hipMemcpyAsync(B.res_out->p, P->S.res, sizeof(double) * ns, hipMemcpyDeviceToDevice,
+                         stream()));
+  sync();
+  if (!ok || !finished) {
+    if (ok) set_error(LAP_ERR_NUMERIC, "pcg_batched: lockstep stop flags never set");
+    return -1.f;
+  }
   return ms;
 }
 
@@ -308,10 +734,22 @@ int lap_pcg_batched_tune(int nsys, const int *row_off, int n_total, int nnz_tota
       win = &c;
     }
   }
-  B->tuned_bs = win->bs;
-  B->tuned_tpr = win->tpr;
-  if (best_block) *best_block = win->bs;
-  if (best_tpr) *best_tpr = win->tpr;
+  int bs_win = win->bs, tpr_win = win->tpr;
+  {  // the lockstep mode on the same trial
+    launch_fill(n_total, 0.0, xs);
+    const float w = run_lockstep(*B, ia, ja, a, d_r, xs, b, 0.0, trial_iters);  // warm
+    launch_fill(n_total, 0.0, xs);
+    const float ms = w < 0.f ? -1.f : run_lockstep(*B, ia, ja, a, d_r, xs, b, 0.0, trial_iters);
+    if (ms < 0.f) return lap_last_status() != LAP_OK ? lap_last_status() : LAP_ERR_HIP;
+    if (ms < best) {
+      bs_win = -1;
+      tpr_win = 0;
+    }
+  }
+  B->tuned_bs = bs_win;
+  B->tuned_tpr = tpr_win;
+  if (best_block) *best_block = bs_win;
+  if (best_tpr) *best_tpr = tpr_win;
   return lap_last_status();
 }
 
@@ -333,12 +771,18 @@ int lap_pcg_batched(int nsys, const int *row_off, int n_total, int nnz_total, co
   }
   if (block == 0) block = 256;
   if (tpr == 0) tpr = 1;
-  const CfgEntry *c = find_cfg(block, tpr);
-  if (!c) {
-    set_error(LAP_ERR_ARG, "pcg_batched: no kernel for block=%d tpr=%d", block, tpr);
-    return LAP_ERR_ARG;
+  float ms = 0.f;
+  if (block == -1) {
+    ms = run_lockstep(*B, ia, ja, a, d_r, x, b, tol, maxit);
+    if (ms < 0.f) return lap_last_status() != LAP_OK ? lap_last_status() : LAP_ERR_HIP;
+  } else {
+    const CfgEntry *c = find_cfg(block, tpr);
+    if (!c) {
+      set_error(LAP_ERR_ARG, "pcg_batched: no kernel for block=%d tpr=%d", block, tpr);
+      return LAP_ERR_ARG;
+    }
+    ms = run_cfg(*c, *B, row_off, ia, ja, a, d_r, x, b, tol, maxit);
   }
-  const float ms = run_cfg(*c, *B, row_off, ia, ja, a, d_r, x, b, tol, maxit);
   if (gpu_ms) *gpu_ms = ms;
   sync();
   if (iters) LAP_HIP(hipMemcpy(iters, B->it_out->p, sizeof(int) * (size_t)nsys,

@@ -41,7 +41,7 @@ def centred(v):
     return v - v.mean()
 
 
-@pytest.mark.parametrize("block,tpr", [(256, 1), (512, 2), (1024, 4), (0, 0)])
+@pytest.mark.parametrize("block,tpr", [(256, 1), (512, 2), (1024, 4), (-1, 0), (0, 0)])
 def test_batched_matches_per_system_oracle(orc, block, tpr):
     print(f"[case] batched block={block} tpr={tpr}", flush=True)
     systems, rhs = make_batch()
@@ -74,13 +74,15 @@ def test_batched_matches_per_system_oracle(orc, block, tpr):
 
 
 def test_batched_tpr1_rows_are_reference_order(orc):
-    """With one thread per row and maxit-bounded runs the iterates follow the
-    oracle's (only the dot-product order differs)."""
+    """With one thread per row (or the lockstep mode's CSR-order SpMV) and
+    maxit-bounded runs the iterates follow the oracle's (only the dot-product
+    order differs)."""
     systems, rhs = make_batch(seed=9, count=6)
     B = S.Batch.from_systems(systems, rhs)
-    x, it, fl, _, _, _ = S.pcg_batched(B, tol=1e-30, maxit=12, block=256, tpr=1)
-    for k, (A, b) in enumerate(zip(systems[:-2], rhs[:-2])):
-        xo, ito, _, _ = orc.cg(A, b, "line_jacobi", tol=1e-30, maxit=12)
-        assert it[k] == ito == 13
-        xk = B.split(x)[k].cpu().numpy()
-        assert np.linalg.norm(xk - xo) <= 1e-10 * np.linalg.norm(xo)
+    for block, tpr in ((256, 1), (-1, 0)):  # persistent, lockstep
+        x, it, fl, _, _, _ = S.pcg_batched(B, tol=1e-30, maxit=12, block=block, tpr=tpr)
+        for k, (A, b) in enumerate(zip(systems[:-2], rhs[:-2])):
+            xo, ito, _, _ = orc.cg(A, b, "line_jacobi", tol=1e-30, maxit=12)
+            assert it[k] == ito == 13
+            xk = B.split(x)[k].cpu().numpy()
+            assert np.linalg.norm(xk - xo) <= 1e-10 * np.linalg.norm(xo), (block, k)
