@@ -97,6 +97,25 @@ def block_iteration_bytes(n: int, nnz: int, k: int) -> int:
     return 12 * nnz + 4 * (n + 1) + 104 * k * n
 
 
+def bench_preconditioners(S, torch, A, Ah, b, tol: float) -> dict:
+    """The same 256^3 system with the IC0 and GS_std preconditioners (M = K =
+    1): setup (host analysis + device factorisation) and one timed solve."""
+    out = {}
+    for kind in ("ichol", "GS_std"):
+        t0 = time.perf_counter()
+        P = S.make_preconditioner(A, Ah, kind)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = S.pcg(A, b, P, tol=tol, maxit=25000)
+        wall = time.perf_counter() - t0
+        out[kind] = {"value": wall, "unit": "s", "iterations": r.iterations, "flag": r.flag,
+                     "device_ms": r.gpu_ms, "setup_s": setup,
+                     "ms_per_iteration": r.gpu_ms / max(r.iterations, 1)}
+        del P
+    return out
+
+
 def bench_block(S, torch, np, side: int = 128, k: int = 64, tol: float = 1e-10) -> dict:
     """k-RHS block PCG (line_jacobi) on side^3 + 1e-8 I against k separate
     single-RHS solves of the same columns."""
@@ -366,14 +385,16 @@ def main() -> int:
         log(f"mixed: {m.iterations} iterations, {m.refinements} refinements, wall "
             f"{mwall * 1e3:.1f} ms, speedup {wall / mwall:.2f}x, err vs fp64 {err:.1e}")
     if not args.no_extras:
-        for name, fn in (("block", lambda: bench_block(S, torch, np)),
+        for name, fn in (("preconditioners",
+                          lambda: bench_preconditioners(S, torch, A, Ah, b, args.tol)),
+                         ("block", lambda: bench_block(S, torch, np)),
                          ("batched", lambda: bench_batched(S, torch, np)),
                          ("adjoint", lambda: bench_adjoint(torch, np))):
             t0 = time.perf_counter()
             try:
                 out[name] = fn()
-                log(f"{name}: {out[name]['value'] * 1e3:.1f} ms "
-                    f"(measured in {time.perf_counter() - t0:.1f}s)")
+                log(f"{name}: {json.dumps(out[name])} (measured in "
+                    f"{time.perf_counter() - t0:.1f}s)")
             except Exception as exc:  # reported, never hides the main line
                 out[name] = {"error": f"{type(exc).__name__}: {exc}"}
                 log(f"{name} failed: {exc}")
