@@ -161,6 +161,7 @@ int lap_set_option(const char *name, int value) {
   else if (!strcmp(name, "spmv_variant")) slot = &o.spmv_variant;
   else if (!strcmp(name, "graph_iters")) slot = &o.graph_iters;
   else if (!strcmp(name, "use_graph")) slot = &o.use_graph;
+  else if (!strcmp(name, "fuse_jacobi")) slot = &o.fuse_jacobi;
   if (!slot) return -1;
   if ((!strcmp(name, "spmv_walk") && (value < 0 || value > 2)) ||
       (!strcmp(name, "vec_grid") && (value < 1 || value > kMaxPartialBlocks / 2))) {

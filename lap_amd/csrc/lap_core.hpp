@@ -176,6 +176,7 @@ struct Options {
   int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;
+  int fuse_jacobi = 0;   // Jacobi PCG: p update fused into the next SpMV
 };
 Options &options();
 

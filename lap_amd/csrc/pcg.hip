@@ -35,9 +35,10 @@ struct PcgCtl {
   double rho;       // r_k.w_k of the iteration in flight (alpha = rho / p.q)
   double rho_prev;  // generic path: rho of the previous iteration
   double tolrel, res0, res, pend_rho;
+  double alpha_x;   // fused Jacobi: alpha of the x update still to apply
   int iter;         // finished iterations, as seen by the kernels
   int maxit, done, flag;
-  int pend_iter, pend_valid, last_iter, pad;
+  int pend_iter, pend_valid, last_iter, pend_done;
 };
 
 enum PrecKind { P_NONE, P_JACOBI, P_IT_JACOBI, P_GS_STD, P_GS_IT, P_GS_IT2, P_ICHOL, P_UNKNOWN };
@@ -140,7 +141,10 @@ __global__ __launch_bounds__(BS) void k_rho_commit(const double *part, int cnt, 
   if (ctl->done) return;
   __shared__ double red[BS / kWave];
   const double rho = sum_partials<BS>(part, cnt, 1, red);
-  if (threadIdx.x == 0) ctl->rho = rho;
+  if (threadIdx.x == 0) {
+    ctl->rho = rho;
+    ctl->pend_rho = rho;  // the fused iteration's K_B' reads rho from here
+  }
 }
 
 struct PqEpi {
@@ -251,6 +255,131 @@ __global__ __launch_bounds__(BS) void k_jac_p(int n, const double *__restrict__ 
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
     const double wi = d_r[i] * r[i];
     p[i] = p[i] * beta + wi;  // p *= beta; p += 1.0*w  (cg.c:80-81)
+  }
+}
+
+// ------------------------------------------------- fused Jacobi iteration
+//
+// Option fuse_jacobi: an iteration is two launches instead of three.
+//   K_AC(t+1): the convergence test of iteration t and beta from K_B(t)'s
+//     partials; p_{t+1} = beta p_t + d_r r computed on the fly wherever the
+//     SpMV gathers it (and written once per row to the other p buffer);
+//     q = A p_{t+1} with p.q partials; and the x update of iteration t,
+//     x += alpha_t p_t, deferred from K_B(t) so p_t is read once.  If
+//     iteration t stops, K_AC only applies that x update.
+//   K_B'(t+1): alpha = rho/(p.q); r -= alpha q; r.w and r.r partials.
+// The arithmetic of every value is the unfused path's (same operations, same
+// order), so results are bitwise identical; only the traffic changes.
+// Control fields: K_AC reads rho, iter, alpha_x (committed by the previous
+// K_B') and has workgroup 0 write pend_rho, pend_iter, the history and the
+// stop state (pend_done); K_B' reads pend_rho and has workgroup 0 commit rho,
+// iter, alpha_x -- and done, once pend_done is set.
+
+struct PnewX {  // p_{t+1}[j], p *= beta; p += 1.0*w with w = d_r .* r (cg.c:80-81)
+  const double *p_old, *r, *d_r;
+  double beta;
+  __device__ double operator()(int j) const { return p_old[j] * beta + d_r[j] * r[j]; }
+};
+
+struct FusedEpi {
+  PnewX pn;
+  double *p_new, *q, *x;
+  double alpha_x, acc;
+  __device__ double init(int) const { return 0.0; }
+  __device__ void store(int row, double s) {
+    const double pv = pn(row);
+    p_new[row] = pv;
+    q[row] = s;
+    acc += pv * s;
+    x[row] += alpha_x * pn.p_old[row];  // x += alpha*p of the previous iteration
+  }
+};
+
+template <int BS, int TILE>
+__global__ __launch_bounds__(BS) void k_jac_fused(int nblk, const int *__restrict__ rowblk,
+                                                  const int *__restrict__ ia,
+                                                  const int *__restrict__ ja,
+                                                  const double *__restrict__ a,
+                                                  const double *__restrict__ p_old,
+                                                  double *__restrict__ p_new,
+                                                  double *__restrict__ q,
+                                                  const double *__restrict__ r,
+                                                  const double *__restrict__ d_r,
+                                                  double *__restrict__ x, PcgCtl *ctl,
+                                                  double *hist, const double *upd_part,
+                                                  int upd_cnt, double *part, int n, int xcd) {
+  if (ctl->done) return;
+  __shared__ double prod[TILE];
+  __shared__ double red[BS / kWave];
+  const double rz = sum_partials<BS>(upd_part, upd_cnt, 2, red);
+  const double rr = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
+  const int it = ctl->iter + 1;
+  const double res = sqrt(rr);
+  const int flag = stop_flag(res, it, ctl);
+  const double alpha_x = ctl->alpha_x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hist[it] = res;
+    ctl->res = res;
+    ctl->last_iter = it;
+    if (flag >= 0) {
+      ctl->flag = flag;
+      ctl->pend_done = 1;
+    } else {
+      ctl->pend_iter = it;
+      ctl->pend_rho = rz;
+    }
+  }
+  if (flag >= 0) {  // iteration `it` stops: only its x update remains
+    const size_t stride = (size_t)gridDim.x * BS;
+    for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride)
+      x[i] += alpha_x * p_old[i];
+    return;
+  }
+  const double beta = rz / ctl->rho;
+  FusedEpi epi{PnewX{p_old, r, d_r, beta}, p_new, q, x, alpha_x, 0.0};
+  const BlockWalk w = block_walk(nblk, xcd);
+  for (int b = w.begin; b < w.end; b += w.step)
+    spmv_block_rows_fx<BS, TILE>(b, rowblk, ia, ja, a, epi.pn, prod, red, epi);
+  const double t = block_sum<BS>(epi.acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// K_B': alpha = pend_rho/(p.q); r -= alpha q; partials of r.(d_r r), r.r
+template <int BS>
+__global__ __launch_bounds__(BS) void k_jac_rupd(int n, double *__restrict__ r,
+                                                 const double *__restrict__ q,
+                                                 const double *__restrict__ d_r, PcgCtl *ctl,
+                                                 const double *pq_part, int pq_cnt,
+                                                 double *partials) {
+  if (ctl->done) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (ctl->pend_done) {
+    if (first) ctl->done = 1;
+    return;
+  }
+  __shared__ double red[BS / kWave];
+  const double pq = sum_partials<BS>(pq_part, pq_cnt, 1, red);
+  const double rho = ctl->pend_rho;
+  const double alpha = rho / pq, nalpha = -1.0 * alpha;
+  double rz = 0.0, rr = 0.0;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+    const double ri = r[i] + nalpha * q[i];
+    r[i] = ri;
+    const double wi = d_r[i] * ri;
+    rz += ri * wi;
+    rr += ri * ri;
+  }
+  const double t_rz = block_sum<BS>(rz, red);
+  const double t_rr = block_sum<BS>(rr, red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = t_rz;
+    partials[2 * blockIdx.x + 1] = t_rr;
+  }
+  if (first) {
+    ctl->rho = rho;
+    ctl->iter = ctl->pend_iter;
+    ctl->alpha_x = alpha;
   }
 }
 
@@ -624,16 +753,33 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   LAP_HIP(hipEventRecord(ev0, stream()));
   launch_init(gA, A, ia, ja, a, x, b, r, R0);
   const bool jac = o.kind == P_JACOBI;
+  const bool fused = jac && options().fuse_jacobi;
+  // fused Jacobi: p alternates between two buffers (p_t is read while
+  // p_{t+1} is written); p_1 goes to pp[1]
+  double *pp[2] = {p, fused ? scratch_vec("pcg_p2", (size_t)n) : p};
   if (jac) {
-    k_start<kBlock, true><<<gV, kBlock, 0, stream()>>>(n, R0, (int)gA, tol, r, pd->d_r, p, ctl,
-                                                       hist, R1);
+    k_start<kBlock, true><<<gV, kBlock, 0, stream()>>>(n, R0, (int)gA, tol, r, pd->d_r, pp[1],
+                                                       ctl, hist, R1);
     k_rho_commit<kBlock><<<1, kBlock, 0, stream()>>>(R1, (int)gV, ctl);
+    if (fused) {  // iteration 1: q = A p_1, then K_B'
+      launch_spmv_pq(gA, A, ia, ja, a, pp[1], q, ctl, R2);
+      k_jac_rupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA, R3);
+    }
   } else {
     k_start<kBlock, false><<<gV, kBlock, 0, stream()>>>(n, R0, (int)gA, tol, r, nullptr, nullptr,
                                                         ctl, hist, nullptr);
   }
+  int s_iter = 0;  // fused: iteration index within a graph (buffer parity)
   auto iteration = [&]() {
-    if (jac) {
+    if (fused) {
+      const double *p_old = pp[(s_iter + 1) & 1];
+      double *p_new = pp[s_iter & 1];
+      k_jac_fused<kBlock, kSpmvTile><<<gA, kBlock, 0, stream()>>>(
+          A.nblocks, A.rowblk->as<int>(), ia, ja, a, p_old, p_new, q, r, pd->d_r, x, ctl, hist,
+          R3, (int)gV, R2, n, options().spmv_walk);
+      k_jac_rupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA, R3);
+      ++s_iter;
+    } else if (jac) {
       launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2);
       k_jac_update<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, r, p, q, pd->d_r, ctl, R2,
                                                         (int)gA, R3);
@@ -648,13 +794,14 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
     }
   };
   // launches per iteration -> iterations per graph
-  int per_iter = jac ? 3 : 6;
+  int per_iter = fused ? 2 : jac ? 3 : 6;
   if (o.kind == P_ICHOL) per_iter += o.ic->lplan->nlevels + o.ic->uplan->nlevels;
   if (o.kind == P_GS_STD) per_iter += (o.Lt->nlevels + o.Ut->nlevels + 4) * o.k;
   if (o.kind == P_IT_JACOBI || o.kind == P_GS_IT || o.kind == P_GS_IT2)
     per_iter += 8 * o.m * (o.k + 2);
   int G = std::max(1, std::min(options().graph_iters, 4096 / per_iter));
   G = std::min(G, maxit + 2);
+  if (fused) G = std::max(2, G & ~1);  // even: every batch starts on the same p buffer
 
   int status = LAP_OK;
   int *flags = pinned_flags();

@@ -86,6 +86,35 @@ __device__ __forceinline__ void spmv_block_rows(int b, const int *__restrict__ r
   __syncthreads();
 }
 
+// Variant 0 body with x given by a functor xf(j) (a value computed from
+// several vectors at the gathered index).  Same products and row order as
+// spmv_block_rows with alpha = 1.
+template <int BS, int TILE, class Epi, class XF>
+__device__ __forceinline__ void spmv_block_rows_fx(int b, const int *__restrict__ rowblk,
+                                                   const int *__restrict__ ia,
+                                                   const int *__restrict__ ja,
+                                                   const double *__restrict__ a, const XF &xf,
+                                                   double *prod, double *red, Epi &epi) {
+  const int r0 = rowblk[b], r1 = rowblk[b + 1];
+  const int e0 = ia[r0], e1 = ia[r1];
+  const int cnt = e1 - e0;
+  if (cnt > TILE) {
+    double s = 0.0;
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += BS) s += (1.0 * a[e]) * xf(ja[e]);
+    const double t = block_sum<BS>(s, red);
+    if (threadIdx.x == 0) epi.store(r0, epi.init(r0) + t);
+    return;
+  }
+#pragma unroll 4
+  for (int k = threadIdx.x; k < cnt; k += BS) {
+    const int e = e0 + k;
+    prod[k] = (1.0 * a[e]) * xf(ja[e]);
+  }
+  __syncthreads();
+  spmv_row_sums<BS>(r0, r1, e0, ia, prod, epi);
+  __syncthreads();
+}
+
 // Variants 1/2: a persistent walk over the row blocks [w.begin, w.end) with
 // stride w.step, register-staged loads and prefetched block bounds.
 template <int BS, int TILE, bool NT, class Epi>
