@@ -318,13 +318,22 @@ double lap_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double
 double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
                           const double *p, double *q, int reps);
 
+/* Same for the fused Jacobi iteration's first kernel (option fuse_jacobi):
+ * p_new = beta p_old + d_r r in the SpMV gather, q = A p_new with p.q, and
+ * x += alpha p_old; timed with beta = 0 and alpha = 0.5. */
+double lap_pcg_fused_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
+                           const double *p_old, double *p_new, double *q, const double *r,
+                           const double *d_r, double *x, int reps);
+
 /* Library configuration knobs (for tuning sweeps): name/value pairs.
  * "spmv_rows" (rows per SpMV block), "spmv_walk" (row-block order of the
  * persistent SpMV grid: 0 cyclic, 1 per-XCD eighths, 2 contiguous chunk per
  * workgroup), "spmv_variant" (SpMV body, 0-4), "vec_grid" (workgroups of
  * the vector kernels), "graph_iters" (iterations per HIP graph launch),
- * "use_graph".  Results do not depend on any of them except vec_grid,
- * which sets the dot-product summation order.  Returns the previous value,
+ * "use_graph", "fuse_jacobi" (Jacobi PCG in two launches per iteration:
+ * the p update computed in the next SpMV's gather, the x update deferred;
+ * bitwise the same results).  Results do not depend on any of them except
+ * vec_grid, which sets the dot-product summation order.  Returns the previous value,
  * or -1 for an unknown name. */
 int lap_set_option(const char *name, int value);
 
