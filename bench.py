@@ -33,12 +33,15 @@ def spmv_bytes(n: int, nnz: int) -> int:
     return 12 * nnz + 4 * (n + 1) + 16 * n
 
 
-def jacobi_iteration_bytes(n: int, nnz: int, fused: bool = False) -> int:
-    """SpMV + 88 B per row of vector traffic (x, r, p, q, d_r reads; x, r
-    writes; p update).  Fused iteration: 12 nnz + 4(n+1) + 88 n."""
-    if fused:
+def jacobi_iteration_bytes(n: int, nnz: int, variant: int = 0) -> int:
+    """Bytes of one Jacobi PCG iteration.  Variant 0: SpMV + 88 B per row
+    (K_B reads x, r, p, q, 1/diag and writes x, r; K_C reads p, r, 1/diag and
+    writes p).  Variant 1 (x updated in K_C, where p is read anyway): SpMV +
+    80 B per row.  Variant 2 (p update in the SpMV gather): 12 nnz + 4(n+1) +
+    88 B per row."""
+    if variant == 2:
         return 12 * nnz + 4 * (n + 1) + 88 * n
-    return spmv_bytes(n, nnz) + 88 * n
+    return spmv_bytes(n, nnz) + (80 if variant == 1 else 88) * n
 
 
 def fused_kernel_bytes(n: int, nnz: int) -> int:
@@ -276,9 +279,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--fuse", type=int, default=0,
-                    help="Jacobi PCG with the p update fused into the next SpMV (1) or the "
-                         "three-kernel iteration (0)")
+    ap.add_argument("--jacobi-variant", type=int, default=1,
+                    help="Jacobi iteration: 0 three launches, x beside r; 1 three launches, "
+                         "x beside p; 2 two launches, p update in the SpMV gather")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the multi-RHS, batched and adjoint measurements")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -307,9 +310,9 @@ def main() -> int:
     A = S.DeviceCSR.from_csr(Ah)
     b = S.to_dev(b_h)
     P = S.make_preconditioner(A, Ah, args.prec)
-    fused = bool(args.fuse) and args.prec == "line_jacobi"
-    S.set_option("fuse_jacobi", 1 if fused else 0)
-    log(f"setup done (fused Jacobi iteration: {fused})")
+    S.set_option("jacobi_variant", args.jacobi_variant)
+    fused = args.jacobi_variant == 2 and args.prec == "line_jacobi"
+    log(f"setup done (Jacobi iteration variant {args.jacobi_variant})")
 
     for w in range(args.warmup):
         r = S.pcg(A, b, P, tol=args.tol, maxit=25000)
@@ -341,7 +344,7 @@ def main() -> int:
         bytes_spmv = spmv_bytes(A.n, A.nnz)
         kname = "k_spmv_pq (fused CSR SpMV q=A p with p.q)"
     achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9
-    it_bytes = jacobi_iteration_bytes(A.n, A.nnz, fused)
+    it_bytes = jacobi_iteration_bytes(A.n, A.nnz, args.jacobi_variant if args.prec == "line_jacobi" else 0)
     log(f"dominant kernel {spmv_ms:.4f} ms -> {achieved:.0f} GB/s")
 
     out = {
@@ -361,7 +364,8 @@ def main() -> int:
             "workload": (f"3D 7-point graph Laplacian {args.size}^3 + 1e-8*I, b~N(0,1) seed "
                          f"12345 mean-subtracted, FP64 PCG ({args.prec}) from x0=0 to "
                          f"rel-res {args.tol:g}"),
-            "n": A.n, "nnz": A.nnz, "preconditioner": args.prec, "fused_iteration": fused,
+            "n": A.n, "nnz": A.nnz, "preconditioner": args.prec,
+            "jacobi_variant": args.jacobi_variant,
             "iterations": its, "flag": sorted(flags),
         },
         "device_ms_per_solve": gpu_ms,

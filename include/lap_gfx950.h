@@ -318,7 +318,7 @@ double lap_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double
 double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
                           const double *p, double *q, int reps);
 
-/* Same for the fused Jacobi iteration's first kernel (option fuse_jacobi):
+/* Same for the two-launch Jacobi iteration's first kernel (jacobi_variant 2):
  * p_new = beta p_old + d_r r in the SpMV gather, q = A p_new with p.q, and
  * x += alpha p_old; timed with beta = 0 and alpha = 0.5. */
 double lap_pcg_fused_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
@@ -330,9 +330,10 @@ double lap_pcg_fused_timed(int n, int nnz, const int *ia, const int *ja, const d
  * persistent SpMV grid: 0 cyclic, 1 per-XCD eighths, 2 contiguous chunk per
  * workgroup), "spmv_variant" (SpMV body, 0-4), "vec_grid" (workgroups of
  * the vector kernels), "graph_iters" (iterations per HIP graph launch),
- * "use_graph", "fuse_jacobi" (Jacobi PCG in two launches per iteration:
- * the p update computed in the next SpMV's gather, the x update deferred;
- * bitwise the same results).  Results do not depend on any of them except
+ * "use_graph", "jacobi_variant" (the Jacobi PCG iteration: 0 three launches
+ * with x updated beside r; 1, the default, three launches with x updated
+ * beside p, which reads p once less; 2 two launches, the p update computed
+ * in the next SpMV's gather).  Results do not depend on any of them except
  * vec_grid, which sets the dot-product summation order.  Returns the previous value,
  * or -1 for an unknown name. */
 int lap_set_option(const char *name, int value);

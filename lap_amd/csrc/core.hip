@@ -161,11 +161,15 @@ int lap_set_option(const char *name, int value) {
   else if (!strcmp(name, "spmv_variant")) slot = &o.spmv_variant;
   else if (!strcmp(name, "graph_iters")) slot = &o.graph_iters;
   else if (!strcmp(name, "use_graph")) slot = &o.use_graph;
-  else if (!strcmp(name, "fuse_jacobi")) slot = &o.fuse_jacobi;
+  else if (!strcmp(name, "jacobi_variant")) slot = &o.jacobi_variant;
   if (!slot) return -1;
   if ((!strcmp(name, "spmv_walk") && (value < 0 || value > 2)) ||
       (!strcmp(name, "vec_grid") && (value < 1 || value > kMaxPartialBlocks / 2))) {
     set_error(LAP_ERR_ARG, "option %s: value %d out of range", name, value);
+    return *slot;
+  }
+  if (!strcmp(name, "jacobi_variant") && (value < 0 || value > 2)) {
+    set_error(LAP_ERR_ARG, "jacobi_variant must be 0, 1 or 2");
     return *slot;
   }
   if (!strcmp(name, "spmv_variant") && (value < 0 || value > 4)) {

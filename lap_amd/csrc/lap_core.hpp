@@ -176,7 +176,9 @@ struct Options {
   int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;
-  int fuse_jacobi = 0;   // Jacobi PCG: p update fused into the next SpMV
+  int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
+                           // 1 three kernels, x in K_C; 2 two kernels (p in the
+                           // SpMV gather)
 };
 Options &options();
 

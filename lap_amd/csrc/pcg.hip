@@ -120,6 +120,7 @@ __global__ __launch_bounds__(BS) void k_start(int n, const double *rr_part, int 
     if (stop) {
       ctl->flag = isfinite(res0) ? 0 : 2;
       ctl->done = 1;
+      ctl->pend_done = 1;
     }
   }
   if (!JAC || stop) return;
@@ -169,6 +170,10 @@ __global__ __launch_bounds__(BS) void k_spmv_pq(int nblk, const int *__restrict_
                                                 double *__restrict__ q, PcgCtl *ctl,
                                                 double *partials, int xcd) {
   if (ctl->done) return;
+  if (ctl->pend_done) {  // the previous kernel stopped the solve: commit it
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = 1;
+    return;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->pend_valid) {
     ctl->iter = ctl->pend_iter;
     ctl->rho = ctl->pend_rho;
@@ -258,9 +263,90 @@ __global__ __launch_bounds__(BS) void k_jac_p(int n, const double *__restrict__ 
   }
 }
 
+// ------------------------------------- Jacobi iteration, x update in K_C
+//
+// jacobi_variant 1: the x update x += alpha p moves from K_B to K_C, which
+// reads p anyway for p = beta p + d_r r, so p is read once less per
+// iteration.  K_B leaves alpha in ctl->alpha_x; when K_C finds that the
+// iteration stops it applies only the x update, and its workgroup 0 marks
+// pend_done (committed to done by the next K_A), so that no workgroup of K_C
+// can skip its rows' x update by seeing `done` early.  Same operations on
+// the same values as the three-kernel iteration: bitwise the same results.
+
+// K_B: alpha = rho/(p.q); r -= alpha q; partials of r.(d_r r) and r.r
+template <int BS>
+__global__ __launch_bounds__(BS) void k_jac_update_nx(int n, double *__restrict__ r,
+                                                      const double *__restrict__ q,
+                                                      const double *__restrict__ d_r,
+                                                      PcgCtl *ctl, const double *pq_part,
+                                                      int pq_cnt, double *partials) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  const double pq = sum_partials<BS>(pq_part, pq_cnt, 1, red);
+  const double alpha = ctl->rho / pq, nalpha = -1.0 * alpha;
+  double rz = 0.0, rr = 0.0;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+    const double ri = r[i] + nalpha * q[i];
+    r[i] = ri;
+    const double wi = d_r[i] * ri;
+    rz += ri * wi;
+    rr += ri * ri;
+  }
+  const double t_rz = block_sum<BS>(rz, red);
+  const double t_rr = block_sum<BS>(rr, red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = t_rz;
+    partials[2 * blockIdx.x + 1] = t_rr;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->alpha_x = alpha;
+}
+
+// K_C: convergence test and beta; x += alpha p, then p = beta p + d_r r
+template <int BS>
+__global__ __launch_bounds__(BS) void k_jac_px(int n, double *__restrict__ x,
+                                               const double *__restrict__ r,
+                                               const double *__restrict__ d_r,
+                                               double *__restrict__ p, PcgCtl *ctl,
+                                               double *hist, const double *part, int cnt) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  const double rz = sum_partials<BS>(part, cnt, 2, red);
+  const double rr = sum_partials<BS>(part + 1, cnt, 2, red);
+  const int it = ctl->iter + 1;
+  const double res = sqrt(rr);
+  const int flag = stop_flag(res, it, ctl);
+  const double alpha = ctl->alpha_x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hist[it] = res;
+    ctl->res = res;
+    ctl->last_iter = it;
+    if (flag >= 0) {
+      ctl->flag = flag;
+      ctl->pend_done = 1;
+    } else {
+      ctl->pend_iter = it;
+      ctl->pend_rho = rz;
+      ctl->pend_valid = 1;
+    }
+  }
+  const size_t stride = (size_t)gridDim.x * BS;
+  if (flag >= 0) {  // the last iteration: only its x update remains
+    for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride)
+      x[i] += alpha * p[i];
+    return;
+  }
+  const double beta = rz / ctl->rho;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+    const double pi = p[i];
+    x[i] += alpha * pi;                  // x += alpha*p    (cg.c:87)
+    p[i] = pi * beta + d_r[i] * r[i];    // p *= beta; p += 1.0*w  (cg.c:80-81)
+  }
+}
+
 // ------------------------------------------------- fused Jacobi iteration
 //
-// Option fuse_jacobi: an iteration is two launches instead of three.
+// jacobi_variant 2: an iteration is two launches instead of three.
 //   K_AC(t+1): the convergence test of iteration t and beta from K_B(t)'s
 //     partials; p_{t+1} = beta p_t + d_r r computed on the fly wherever the
 //     SpMV gathers it (and written once per row to the other p buffer);
@@ -753,7 +839,8 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   LAP_HIP(hipEventRecord(ev0, stream()));
   launch_init(gA, A, ia, ja, a, x, b, r, R0);
   const bool jac = o.kind == P_JACOBI;
-  const bool fused = jac && options().fuse_jacobi;
+  const int jv = jac ? options().jacobi_variant : 0;
+  const bool fused = jv == 2, xin_c = jv == 1;
   // fused Jacobi: p alternates between two buffers (p_t is read while
   // p_{t+1} is written); p_1 goes to pp[1]
   double *pp[2] = {p, fused ? scratch_vec("pcg_p2", (size_t)n) : p};
@@ -779,6 +866,11 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
           R3, (int)gV, R2, n, options().spmv_walk);
       k_jac_rupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA, R3);
       ++s_iter;
+    } else if (xin_c) {
+      launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2);
+      k_jac_update_nx<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA,
+                                                           R3);
+      k_jac_px<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, r, pd->d_r, p, ctl, hist, R3, (int)gV);
     } else if (jac) {
       launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2);
       k_jac_update<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, r, p, q, pd->d_r, ctl, R2,
@@ -794,6 +886,9 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
     }
   };
   // launches per iteration -> iterations per graph
+  // variants that stop through pend_done are polled there (done follows in
+  // the next launch)
+  const int *poll = (fused || xin_c) ? &ctl->pend_done : done;
   int per_iter = fused ? 2 : jac ? 3 : 6;
   if (o.kind == P_ICHOL) per_iter += o.ic->lplan->nlevels + o.ic->uplan->nlevels;
   if (o.kind == P_GS_STD) per_iter += (o.Lt->nlevels + o.Ut->nlevels + 4) * o.k;
@@ -835,7 +930,7 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
         } else {
           for (int s = 0; s < G; ++s) iteration();
         }
-        LAP_HIP(hipMemcpyAsync(flags + (k & 1), done, sizeof(int), hipMemcpyDeviceToHost,
+        LAP_HIP(hipMemcpyAsync(flags + (k & 1), poll, sizeof(int), hipMemcpyDeviceToHost,
                                stream()));
         LAP_HIP(hipEventRecord(pe[k & 1], stream()));
       }

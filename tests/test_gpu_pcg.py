@@ -224,27 +224,29 @@ def test_pcg_256cubed_matches_reference():
 
 
 @pytest.mark.parametrize("pname", ["grid3d_16", "graph_big"])
-def test_fused_jacobi_iteration_is_bitwise_the_unfused_one(pname):
-    """Option fuse_jacobi (p update fused into the next SpMV, x update
-    deferred) changes only the traffic: x, the iteration count and the
-    residual history are bit-identical to the three-kernel iteration, at a
-    fixed iteration count, at convergence, for maxit = 0 and for b = 0."""
-    print(f"[case] fused jacobi problem={pname}", flush=True)
+def test_jacobi_variants_are_bitwise_identical(pname):
+    """jacobi_variant 0 (x updated beside r), 1 (x updated beside p, the
+    default) and 2 (p update in the next SpMV's gather, two launches) change
+    only the traffic: x, the iteration count and the residual history are
+    bit-identical, at a fixed iteration count, at convergence, for maxit = 0
+    and 1, and for b = 0."""
+    print(f"[case] jacobi variants problem={pname}", flush=True)
     Ah = PROBLEMS[pname]()
     A = S.DeviceCSR.from_csr(Ah)
     P = S.make_preconditioner(A, Ah, "line_jacobi")
     b = S.to_dev(rhs_normal(Ah.n, 77))
     cases = [(b, 1e-30, 25), (b, 1e-10, 5000), (b, 1e-10, 0), (b, 1e-10, 1),
              (torch.zeros_like(b), 1e-10, 100)]
-    prev = S.set_option("fuse_jacobi", 0)
+    prev = S.set_option("jacobi_variant", 0)
     try:
         for rhs, tol, maxit in cases:
-            S.set_option("fuse_jacobi", 0)
+            S.set_option("jacobi_variant", 0)
             r0 = S.pcg(A, rhs, P, tol=tol, maxit=maxit)
-            S.set_option("fuse_jacobi", 1)
-            r1 = S.pcg(A, rhs, P, tol=tol, maxit=maxit)
-            assert (r1.iterations, r1.flag) == (r0.iterations, r0.flag), (tol, maxit)
-            assert np.array_equal(r1.history, r0.history), (tol, maxit)
-            assert torch.equal(r1.x, r0.x), (tol, maxit)
+            for v in (1, 2):
+                S.set_option("jacobi_variant", v)
+                r1 = S.pcg(A, rhs, P, tol=tol, maxit=maxit)
+                assert (r1.iterations, r1.flag) == (r0.iterations, r0.flag), (v, tol, maxit)
+                assert np.array_equal(r1.history, r0.history), (v, tol, maxit)
+                assert torch.equal(r1.x, r0.x), (v, tol, maxit)
     finally:
-        S.set_option("fuse_jacobi", prev)
+        S.set_option("jacobi_variant", prev)
