@@ -176,6 +176,8 @@ struct Options {
   int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;
+  int sptrsv_mode = 0;     // triangular solves: 0 one launch per level,
+                           // 1 one sync-free launch
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
                            // 1 three kernels, x in K_C; 2 two kernels (p in the
                            // SpMV gather)
@@ -219,12 +221,18 @@ struct TriPlan {
   DevBufPtr order;             // int[n]: rows sorted by level
   int max_level_rows = 0;
   bool has_diag = true;        // every row holds its pivot
+  int nchunks = 0;             // sync-free mode: <= 64 rows of one level each
+  DevBufPtr chunk_ptr;         // int[nchunks + 1], offsets into order
 };
 const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower);
 // y = T^{-1} x (level-scheduled); ctl_done (device int*, may be null) makes
 // every launch a no-op once *ctl_done != 0.
 void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
                       const double *diag, const double *x, double *y, const int *ctl_done);
+// Sync-free mode: nonzero once a dependency wait ran out (the result is then
+// invalid); reset by tri_clear_error.
+int tri_error();
+void tri_clear_error();
 
 // -------------------------------------------------------------- IC0 plan
 struct IcPlan {

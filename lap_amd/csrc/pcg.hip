@@ -829,6 +829,9 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
 
   double *R0 = ws.partials + kR0, *R1 = ws.partials + kR1, *R2 = ws.partials + kR2,
          *R3 = ws.partials + kR3;
+  const bool sync_free = options().sptrsv_mode == 1 &&
+                         (o.kind == P_ICHOL || o.kind == P_GS_STD);
+  if (sync_free) tri_clear_error();
   PcgCtl h0{};
   h0.maxit = maxit;
   LAP_HIP(hipMemcpyAsync(ctl, &h0, sizeof h0, hipMemcpyHostToDevice, stream()));
@@ -964,6 +967,11 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   if (h_hist) LAP_HIP(hipMemcpy(h_hist, hist, sizeof(double) * (size_t)(h.last_iter + 1),
                                hipMemcpyDeviceToHost));
   if (status == LAP_OK && lap_last_status() == LAP_ERR_HIP) status = LAP_ERR_HIP;
+  if (sync_free && tri_error()) {
+    set_error(LAP_ERR_NUMERIC, "pcg: a triangular-solve dependency wait ran out");
+    tri_clear_error();
+    if (status == LAP_OK) status = LAP_ERR_NUMERIC;
+  }
   return status;
 }
 
@@ -1112,6 +1120,10 @@ void prec_function(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, 
   if (!prec_setup(o, prec_data->n, nnzA, ia, ja, a, prec_data)) return;
   prec_launch(o, x, y, nullptr);
   sync();
+  if (options().sptrsv_mode == 1 && tri_error()) {
+    set_error(LAP_ERR_NUMERIC, "prec_function: a triangular-solve dependency wait ran out");
+    tri_clear_error();
+  }
 }
 
 static void prec_named(const char *name, int *ia, int *ja, double *a, int nnzA,

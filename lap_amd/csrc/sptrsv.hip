@@ -7,6 +7,23 @@
 // row's off-diagonal terms in CSR order and divides by the pivot, so every
 // row is computed with exactly the reference's sequential arithmetic
 // (simple_blas.c:46-85) and the result equals the sequential solve bitwise.
+//
+// Sync-free mode (option sptrsv_mode 1): one launch per solve.  Rows are cut
+// into chunks of at most 64 rows of a single level, in level order.  A wave
+// claims 4 consecutive chunks at a time from an atomic ticket and solves
+// them in order, one lane per row.  Before the solve y is filled with a
+// sentinel (a signalling-NaN bit pattern no arithmetic produces); a lane
+// reads each dependency y[c] with 64-bit agent-scope atomic loads until it
+// is not the sentinel, and stores its result with a 64-bit agent-scope
+// atomic store, so the value is its own ready flag and no fence is needed.
+// Progress: a lane only waits for rows of earlier levels, i.e. of earlier
+// chunks, which waves that are already running have claimed (tickets are
+// taken in order), and rows of one chunk never depend on each other.  Every
+// wait is bounded: past the bound (far beyond any real wait) the lane raises
+// an error flag, every other waiting lane gives up as soon as it sees it,
+// and the host reports the failure, so the kernel always ends.  The
+// arithmetic is the level-scheduled kernel's, in the same order: results
+// are bitwise identical.
 #include <algorithm>
 
 #include "lap_core.hpp"
@@ -39,6 +56,91 @@ __global__ __launch_bounds__(kBlock) void k_tri_level(
   // diag is only passed (and only read) for matrices without in-row pivots
   if (!in_row && diag) piv = diag[row];
   y[row] = acc / piv;
+}
+
+constexpr unsigned long long kSentinel = 0x7ff44c4150534e54ull;  // sNaN payload
+constexpr int kSfChunk = 64, kSfClaim = 4;
+constexpr unsigned kSfSpinLimit = 1u << 20;
+
+__device__ __forceinline__ double sf_load(const double *p) {
+  const unsigned long long v = __hip_atomic_load(
+      reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)v);
+}
+
+// y[i] = sentinel for i < n; state[0] (ticket) = 0
+__global__ __launch_bounds__(kBlock) void k_sf_fill(int n, double *__restrict__ y, int *state,
+                                                    const int *__restrict__ done) {
+  if (done && *done) return;
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  unsigned long long *yy = reinterpret_cast<unsigned long long *>(y);
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n; i += stride)
+    yy[i] = kSentinel;
+  if (blockIdx.x == 0 && threadIdx.x == 0) state[0] = 0;
+}
+
+// state[0]: chunk ticket; state[1]: error flag (dependency wait ran out)
+__global__ __launch_bounds__(kBlock) void k_tri_syncfree(
+    int nchunks, const int *__restrict__ chunk_ptr, const int *__restrict__ rows,
+    const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
+    const double *__restrict__ diag, const double *__restrict__ x, double *y, int *state,
+    const int *__restrict__ done) {
+  if (done && *done) return;
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int first = 0;
+    if (lane == 0) first = atomicAdd(&state[0], kSfClaim);
+    first = __builtin_amdgcn_readfirstlane(__shfl(first, 0));
+    if (first >= nchunks) break;
+    const int last = min(first + kSfClaim, nchunks);
+    for (int c = first; c < last; ++c) {
+      const int t = chunk_ptr[c] + lane;
+      if (t >= chunk_ptr[c + 1]) continue;
+      const int row = rows[t];
+      double acc = x[row];
+      double piv = 0.0;
+      bool in_row = false;
+      const int e1 = ia[row + 1];
+      for (int e = ia[row]; e < e1; ++e) {
+        const int col = ja[e];
+        if (col == row) {
+          piv = a[e];
+          in_row = true;
+          continue;
+        }
+        double v = sf_load(y + col);
+        unsigned spins = 0;
+        while (__double_as_longlong(v) == (long long)kSentinel) {
+          if (++spins > kSfSpinLimit ||
+              __hip_atomic_load(&state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_store(&state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v = __builtin_nan("");
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+          v = sf_load(y + col);
+        }
+        acc -= a[e] * v;
+      }
+      if (!in_row && diag) piv = diag[row];
+      const double r = acc / piv;
+      __hip_atomic_store(reinterpret_cast<unsigned long long *>(y + row),
+                         (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+int *sf_state() {  // [ticket, error]
+  static DevBufPtr buf;
+  if (!buf) {
+    buf = dev_alloc(2 * sizeof(int));
+    if (buf) {
+      LAP_HIP(hipMemsetAsync(buf->p, 0, 2 * sizeof(int), stream()));
+      sync();
+    }
+  }
+  return buf ? buf->as<int>() : nullptr;
 }
 
 using Key = std::tuple<const int *, const int *, int, int, bool>;
@@ -102,6 +204,14 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
   for (int l = 0; l < nlev; ++l)
     p->max_level_rows = std::max(p->max_level_rows, p->level_ptr[l + 1] - p->level_ptr[l]);
   p->order = dev_upload(order);
+  // sync-free chunks: <= kSfChunk rows, never spanning two levels
+  std::vector<int> cp;
+  cp.reserve((size_t)n / kSfChunk + (size_t)nlev + 1);
+  for (int l = 0; l < nlev; ++l)
+    for (int t = p->level_ptr[l]; t < p->level_ptr[l + 1]; t += kSfChunk) cp.push_back(t);
+  cp.push_back(n);
+  p->nchunks = (int)cp.size() - 1;
+  p->chunk_ptr = dev_upload(cp);
   return p;
 }
 
@@ -127,9 +237,32 @@ const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower
   return raw;
 }
 
+int tri_error() {
+  int *st = sf_state();
+  if (!st) return 0;
+  int h = 0;
+  LAP_HIP(hipMemcpy(&h, st + 1, sizeof(int), hipMemcpyDeviceToHost));
+  return h;
+}
+
+void tri_clear_error() {
+  int *st = sf_state();
+  if (st) LAP_HIP(hipMemsetAsync(st, 0, 2 * sizeof(int), stream()));
+}
+
 void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
                       const double *diag, const double *x, double *y, const int *done) {
   const int *order = p.order->as<int>();
+  // in place (x == y) needs x intact while y is filled: level launches then
+  int *st = (options().sptrsv_mode == 1 && x != y) ? sf_state() : nullptr;
+  if (st && p.n > 0) {
+    const unsigned gf = grid_for((size_t)p.n, 4, kBlock, 2048);
+    k_sf_fill<<<gf, kBlock, 0, stream()>>>(p.n, y, st, done);
+    const unsigned g = (unsigned)std::max(1, std::min((p.nchunks + 3) / 4, 8 * device_cus()));
+    k_tri_syncfree<<<g, kBlock, 0, stream()>>>(p.nchunks, p.chunk_ptr->as<int>(), order, ia, ja,
+                                               a, diag, x, y, st, done);
+    return;
+  }
   for (int l = 0; l < p.nlevels; ++l) {
     const int cnt = p.level_ptr[l + 1] - p.level_ptr[l];
     const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock);
@@ -163,6 +296,10 @@ static void tri_entry(int n, int nnz, const int *ia, const int *ja, const double
   }
   tri_solve_launch(*p, ia, ja, a, dg, x, y, nullptr);
   sync();
+  if (options().sptrsv_mode == 1 && tri_error()) {
+    set_error(LAP_ERR_NUMERIC, "triangular solve: a dependency wait ran out (result invalid)");
+    tri_clear_error();
+  }
 }
 
 }  // namespace lap

@@ -9,7 +9,7 @@ import torch
 
 from cases import PROBLEMS, star_graph
 from lap_amd import solver as S
-from lap_amd.problems import CSR, augment_lower, augment_upper, split
+from lap_amd.problems import CSR, augment_lower, augment_upper, grid_laplacian, split
 
 pytestmark = pytest.mark.gpu
 
@@ -241,3 +241,32 @@ def test_entry_points_refuse_host_vectors():
     lib.hip_dot(100, S.ptr(d), hp)
     with pytest.raises(L.LapError):
         L.check()
+
+
+@pytest.fixture
+def sync_free():
+    """Triangular solves in sync-free mode (one launch, dependency waits on
+    the values themselves) for the duration of a test."""
+    prev = S.set_option("sptrsv_mode", 1)
+    yield
+    S.set_option("sptrsv_mode", prev)
+
+
+def test_trisolve_sync_free_bitwise(orc, problem, sync_free):
+    """Same bits as the sequential solve (and so as the level-scheduled
+    launches), both conventions, lower and upper."""
+    print(f"[case] sync-free trisolve n={problem.n}", flush=True)
+    test_trisolve_bitwise(orc, problem)
+
+
+@pytest.mark.parametrize("kind", ["GS_std", "ichol"])
+def test_prec_sync_free_bitwise(orc, problem, kind, sync_free):
+    print(f"[case] sync-free prec {kind} n={problem.n}", flush=True)
+    test_prec_bitwise(orc, problem, kind)
+
+
+def test_trisolve_sync_free_large_bitwise(orc, sync_free):
+    """A 48^3 grid (142 levels, ~1700 chunks): every dependency crosses
+    chunks claimed by other waves."""
+    A = grid_laplacian([48, 48, 48], 1e-8)
+    test_trisolve_bitwise(orc, A)

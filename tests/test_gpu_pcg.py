@@ -81,7 +81,8 @@ def test_pcg_matches_golden(name):
 @pytest.mark.parametrize("kind", ["line_jacobi", "GS_std", "ichol", "it_jacobi", "GS_it2", "none"])
 @pytest.mark.parametrize("pname", ["grid3d_16", "graph_big"])
 def test_pcg_equal_iterations_vs_oracle(orc, kind, pname):
-    """Same number of iterations (maxit-bounded): x agrees to 1e-9.  K=1
+    """Same number of iterations (maxit-bounded): x agrees to 1e-9 (or to
+    the reference's own serial/OpenMP spread where CG is unstable).  K=1
     except it_jacobi (K sweeps of Jacobi stay symmetric); GS_std with K>1 is
     not a symmetric preconditioner, so CG on it amplifies any rounding
     difference and is not compared here."""
@@ -95,11 +96,20 @@ def test_pcg_equal_iterations_vs_oracle(orc, kind, pname):
     res = S.pcg(A, S.to_dev(b), P, tol=1e-30, maxit=maxit)
     xo, ito, flo, ho = orc.cg(Ah, b, kind, tol=1e-30, maxit=maxit, m=1, k=k)
     assert (res.iterations, res.flag) == (ito, flo) == (maxit + 1, 1)
-    # only the dot products round differently; CG amplifies that by up to a
-    # few 1e-9 within 25 iterations on the lognormal-weighted graph without
-    # preconditioner, and residuals below ~1e-11 ||r0|| are rounding noise
-    np.testing.assert_allclose(res.history, ho, rtol=1e-8, atol=1e-12 * ho[0])
-    assert rel(res.x.cpu().numpy(), xo) <= 1e-9
+    # Only the dot products round differently.  Without a preconditioner on
+    # the lognormal-weighted graph CG loses orthogonality within 25 iterations
+    # (the residual grows again after iteration 14) and amplifies any rounding
+    # difference: the reference's own serial and OpenMP builds (dot products
+    # summed by 2..16 threads) already differ by 2e-8..5.5e-7 in x there.  The
+    # bar is 1e-9, or twice that spread where it is larger.
+    spread = max(rel(orc.cg(Ah, b, kind, tol=1e-30, maxit=maxit, m=1, k=k, omp=(16, th))[0], xo)
+                 for th in (2, 4, 8, 16))
+    dev_x = rel(res.x.cpu().numpy(), xo)
+    print(f"[diag] fixed-iter {kind}/{pname}: x rel dev {dev_x:.1e}, reference serial/OpenMP "
+          f"spread {spread:.1e}", flush=True)
+    np.testing.assert_allclose(res.history, ho, rtol=max(1e-8, 2 * spread),
+                               atol=1e-12 * ho[0])
+    assert dev_x <= max(1e-9, 2 * spread)
 
 
 @pytest.mark.parametrize("kind", ["line_jacobi", "GS_std", "ichol"])
@@ -250,3 +260,27 @@ def test_jacobi_variants_are_bitwise_identical(pname):
                 assert torch.equal(r1.x, r0.x), (v, tol, maxit)
     finally:
         S.set_option("jacobi_variant", prev)
+
+
+@pytest.mark.parametrize("kind", ["ichol", "GS_std"])
+@pytest.mark.parametrize("pname", ["grid3d_16", "graph_big"])
+def test_sync_free_trisolve_pcg_is_bitwise_the_level_scheduled_one(kind, pname):
+    """sptrsv_mode 1 (one sync-free launch per triangular solve) gives the
+    level-scheduled solve's bits, so whole PCG runs agree exactly."""
+    print(f"[case] sync-free pcg kind={kind} problem={pname}", flush=True)
+    Ah = PROBLEMS[pname]()
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, kind)
+    b = S.to_dev(rhs_normal(Ah.n, 78))
+    prev = S.set_option("sptrsv_mode", 0)
+    try:
+        for tol, maxit in ((1e-30, 20), (1e-10, 5000)):
+            S.set_option("sptrsv_mode", 0)
+            r0 = S.pcg(A, b, P, tol=tol, maxit=maxit)
+            S.set_option("sptrsv_mode", 1)
+            r1 = S.pcg(A, b, P, tol=tol, maxit=maxit)
+            assert (r1.iterations, r1.flag) == (r0.iterations, r0.flag)
+            assert np.array_equal(r1.history, r0.history)
+            assert torch.equal(r1.x, r0.x)
+    finally:
+        S.set_option("sptrsv_mode", prev)
