@@ -267,8 +267,9 @@ int lap_pcg_batched_tune(int nsys, const int *row_off, int n_total, int nnz_tota
  * X and B are n x k row-interleaved (element (i, j) at i*k + j), device
  * memory; k is a power of two <= 64; d_r = 1/diag(A) as pdata.d_r.
  * tile: rows per lane group per step of the SpMM (1, 2 or 4; one lane per
- * column, 64/k row groups per wave), 0 = pick by timing (kept per matrix
- * and k).  hist (host, may be NULL):
+ * column, 64/k row groups per wave; with k = 64 also 8 or 16: rows per wave
+ * step, indices and values loaded by the whole wave and broadcast), 0 = pick
+ * by timing (kept per matrix and k).  hist (host, may be NULL):
  * (maxit+2) x k residual norms, row it = iteration it (0 past a column's
  * own stop).  Per-column outputs
  * (host arrays of k, each may be NULL) as lap_pcg_batched. */
@@ -330,7 +331,9 @@ double lap_pcg_fused_timed(int n, int nnz, const int *ia, const int *ja, const d
  * persistent SpMV grid: 0 cyclic, 1 per-XCD eighths, 2 contiguous chunk per
  * workgroup), "spmv_variant" (SpMV body, 0-4), "vec_grid" (workgroups of
  * the vector kernels), "graph_iters" (iterations per HIP graph launch),
- * "use_graph", "jacobi_variant" (the Jacobi PCG iteration: 0 three launches
+ * "use_graph", "spmm_wg_per_cu" (block CG SpMM workgroups per CU),
+ * "sptrsv_mode" (triangular solves: 0 one launch per level, 1 one
+ * sync-free launch), "jacobi_variant" (the Jacobi PCG iteration: 0 three launches
  * with x updated beside r; 1, the default, three launches with x updated
  * beside p, which reads p once less; 2 two launches, the p update computed
  * in the next SpMV's gather).  Results do not depend on any of them except

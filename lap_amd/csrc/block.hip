@@ -140,6 +140,95 @@ __global__ __launch_bounds__(BS) void k_spmm(int n, const int *__restrict__ ia,
   }
 }
 
+// K = 64, wave-cooperative: one wave = the 64 columns of a row; a step takes
+// RU consecutive rows, whose nonzeros are contiguous in CSR, so their
+// column indices and values arrive in one coalesced load per 64 entries and
+// v_readlane hands each entry to the whole wave.  The gathers of up to 16
+// entries are issued before they are summed; rows are summed in CSR order
+// and written as soon as they end.  Control flow is wave-uniform.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int BS, int RU, int MODE>
+__global__ __launch_bounds__(BS) void k_spmm64(int n, const int *__restrict__ ia,
+                                               const int *__restrict__ ja,
+                                               const double *__restrict__ a,
+                                               const double *__restrict__ P,
+                                               double *__restrict__ Q,
+                                               const double *__restrict__ Bm, double *part,
+                                               const BlkCtl *ctl) {
+  static_assert(RU < 64, "row pointers of a step fit one wave");
+  if (MODE == 1 && ctl->all_done) return;
+  constexpr int K = 64, NW = BS / 64, EB = 16;
+  __shared__ double lds[K * NW];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int wave_id = (int)blockIdx.x * NW + wave, nwaves = (int)gridDim.x * NW;
+  double acc = 0.0;
+  auto finish = [&](int row, double sum) {
+    const size_t o = (size_t)row * K + lane;
+    if (MODE == 2) {
+      const double r = -(sum - Bm[o]);  // r = A x; r += -1*b; r *= -1
+      Q[o] = r;
+      acc += r * r;
+    } else {
+      Q[o] = sum;
+      if (MODE == 1) acc += P[o] * sum;
+    }
+  };
+  for (int base = wave_id * RU; base < n; base += nwaves * RU) {
+    const int nrows = min(RU, n - base);
+    const int ptr_l = lane <= nrows ? ia[base + lane] : 0;
+    const int e_begin = __builtin_amdgcn_readlane(ptr_l, 0);
+    const int e_end = __builtin_amdgcn_readlane(ptr_l, nrows);
+    int j = 0;                                             // row within the step
+    int next = __builtin_amdgcn_readlane(ptr_l, 1);        // end of row j
+    double cur = 0.0;
+    for (int c0 = e_begin; c0 < e_end; c0 += 64) {
+      const int cnt = min(64, e_end - c0);
+      const int jl = lane < cnt ? ja[c0 + lane] : 0;
+      const double al = lane < cnt ? a[c0 + lane] : 0.0;
+      for (int t0 = 0; t0 < cnt; t0 += EB) {
+        double v[EB];
+#pragma unroll
+        for (int u = 0; u < EB; ++u)
+          if (t0 + u < cnt)
+            v[u] = P[(size_t)__builtin_amdgcn_readlane(jl, t0 + u) * K + lane];
+#pragma unroll
+        for (int u = 0; u < EB; ++u) {
+          if (t0 + u < cnt) {
+            const int e = c0 + t0 + u;
+            while (e >= next) {  // row j ends before entry e
+              finish(base + j, cur);
+              cur = 0.0;
+              ++j;
+              next = __builtin_amdgcn_readlane(ptr_l, j + 1);
+            }
+            cur += readlane_f64(al, t0 + u) * v[u];
+          }
+        }
+      }
+    }
+    while (j < nrows) {  // the last row with entries, then empty rows
+      finish(base + j, cur);
+      cur = 0.0;
+      ++j;
+    }
+  }
+  if (MODE == 0) return;
+  lds[lane * NW + wave] = acc;
+  __syncthreads();
+  for (int cc = threadIdx.x; cc < K; cc += BS) {
+    double t = 0.0;
+    for (int w = 0; w < NW; ++w) t += lds[cc * NW + w];
+    part[(size_t)blockIdx.x * K + cc] = t;
+  }
+}
+
 // --------------------------------------------------- column reductions
 
 // One workgroup of kRedThreads: out[j] = sum over b < nparts of
@@ -349,6 +438,18 @@ void spmm_launch(unsigned g, int n, const int *ia, const int *ja, const double *
     k_spmm<kBlock, K, RU, EU, 0><<<g, kBlock, 0, stream()>>>(n, ia, ja, a, P, Q, Bm, part, ctl);
 }
 
+template <int RU>
+void spmm64_launch(unsigned g, int n, const int *ia, const int *ja, const double *a,
+                   const double *P, double *Q, const double *Bm, double *part, const BlkCtl *ctl,
+                   int mode) {
+  if (mode == 2)
+    k_spmm64<kBlock, RU, 2><<<g, kBlock, 0, stream()>>>(n, ia, ja, a, P, Q, Bm, part, ctl);
+  else if (mode == 1)
+    k_spmm64<kBlock, RU, 1><<<g, kBlock, 0, stream()>>>(n, ia, ja, a, P, Q, Bm, part, ctl);
+  else
+    k_spmm64<kBlock, RU, 0><<<g, kBlock, 0, stream()>>>(n, ia, ja, a, P, Q, Bm, part, ctl);
+}
+
 template <int K>
 SpmmFn spmm_for_k(int ru) {
   switch (ru) {
@@ -367,12 +468,19 @@ SpmmFn spmm_for(int k, int ru) {
     case 8: return spmm_for_k<8>(ru);
     case 16: return spmm_for_k<16>(ru);
     case 32: return spmm_for_k<32>(ru);
-    case 64: return spmm_for_k<64>(ru);
+    case 64:
+      if (ru == 8) return spmm64_launch<8>;
+      if (ru == 16) return spmm64_launch<16>;
+      return spmm_for_k<64>(ru);
     default: return nullptr;
   }
 }
 
-bool tile_ok(int ru) { return ru == 1 || ru == 2 || ru == 4; }
+// 1, 2, 4: rows per lane group; 8, 16 (k = 64): rows per wave step with
+// wave-cooperative index loads
+bool tile_ok(int k, int ru) {
+  return ru == 1 || ru == 2 || ru == 4 || (k == 64 && (ru == 8 || ru == 16));
+}
 
 int log2i(int k) {
   int l = 0;
@@ -383,7 +491,10 @@ int log2i(int k) {
 // (ia, ja, a, n, k) -> chosen W
 std::map<std::tuple<const void *, const void *, const void *, int, int>, int> g_tile;
 
-unsigned spmm_grid() { return (unsigned)std::max(1, std::min(1024, 4 * device_cus())); }
+unsigned spmm_grid() {
+  const int per_cu = std::max(1, std::min(16, options().spmm_wg_per_cu));
+  return (unsigned)std::max(1, per_cu * device_cus());
+}
 
 unsigned vec_grid(size_t elems) {
   return (unsigned)std::max<size_t>(
@@ -406,7 +517,8 @@ int pick_tile(int n, const int *ia, const int *ja, const double *a, int k, const
     (void)hipEventDestroy(e0);
     return 0;
   }
-  for (int ru = 1; ru <= 4; ru *= 2) {
+  for (int ru = 1; ru <= 16; ru *= 2) {
+    if (!tile_ok(k, ru)) continue;
     SpmmFn fn = spmm_for(k, ru);
     fn(g, n, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
     LAP_HIP(hipEventRecord(e0, stream()));
@@ -434,8 +546,8 @@ bool block_args(const char *fn, int n, int nnz, const int *ia, const int *ja, co
               kMaxK, n, k);
     return false;
   }
-  if (tile && !tile_ok(tile)) {
-    set_error(LAP_ERR_ARG, "%s: tile (rows per lane group) must be 1, 2 or 4, not %d", fn, tile);
+  if (tile && !tile_ok(k, tile)) {
+    set_error(LAP_ERR_ARG, "%s: tile must be 1, 2 or 4 (or 8, 16 with k = 64), not %d", fn, tile);
     return false;
   }
   return check_device(fn, {{ia, "ia", sizeof(int) * ((size_t)n + 1)},
@@ -493,7 +605,8 @@ extern "C" int lap_pcg_block(int n, int nnz, const int *ia, const int *ja, const
          *Q = scratch_vec("blk_q", (size_t)n * k);
   BlkCtl *ctl =
       reinterpret_cast<BlkCtl *>(scratch_vec("blk_ctl", sizeof(BlkCtl) / sizeof(double) + 1));
-  double *part = scratch_vec("blk_part", (size_t)2 * kMaxK * 1024);
+  const unsigned g = spmm_grid(), gv = vec_grid((size_t)n * k);
+  double *part = scratch_vec("blk_part", (size_t)2 * kMaxK * std::max(g, gv));
   const size_t hist_len = ((size_t)maxit + 2) * (size_t)k;
   if (h_hist && hist_len > (size_t)INT_MAX) {
     set_error(LAP_ERR_ARG, "pcg_block: history of (maxit+2)*k entries is too long");
@@ -506,7 +619,6 @@ extern "C" int lap_pcg_block(int n, int nnz, const int *ia, const int *ja, const
   if (!tile) tile = pick_tile(n, ia, ja, a, k, Bm, Q);
   if (!tile) return LAP_ERR_HIP;
   SpmmFn spmm = spmm_for(k, tile);
-  const unsigned g = spmm_grid(), gv = vec_grid((size_t)n * k);
   const int lk = log2i(k);
 
   hipEvent_t ev0, ev1;

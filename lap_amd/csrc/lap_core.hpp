@@ -176,6 +176,7 @@ struct Options {
   int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;
+  int spmm_wg_per_cu = 4;  // block CG SpMM grid: workgroups per CU
   int sptrsv_mode = 0;     // triangular solves: 0 one launch per level,
                            // 1 one sync-free launch
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;

@@ -51,9 +51,15 @@ def test_spmm_columns_bitwise(orc, pname, k):
     X = np.random.default_rng(k).standard_normal((Ah.n, k))
     Xd = S.to_dev(X)
     want = np.stack([orc.matvec(Ah, X[:, j].copy()) for j in range(k)], axis=1)
-    for ru in (1, 2, 4):  # rows per lane group
-        Y = S.spmm(A, Xd, tile=ru).cpu().numpy()
-        assert np.array_equal(Y, want), (ru, np.abs(Y - want).max())
+    tiles = (1, 2, 4, 8, 16) if k == 64 else (1, 2, 4)  # 8, 16: wave-cooperative (k = 64)
+    for per_cu in (4, 8):  # SpMM workgroups per CU
+        prev = S.set_option("spmm_wg_per_cu", per_cu)
+        try:
+            for ru in tiles:
+                Y = S.spmm(A, Xd, tile=ru).cpu().numpy()
+                assert np.array_equal(Y, want), (per_cu, ru, np.abs(Y - want).max())
+        finally:
+            S.set_option("spmm_wg_per_cu", prev)
     Y = S.spmm(A, Xd).cpu().numpy()  # timed choice
     assert np.array_equal(Y, want)
 
@@ -74,7 +80,7 @@ def test_block_fixed_iterations_vs_oracle(orc, pname):
         assert rel(X[:, j], xo) <= 1e-9, j
 
 
-@pytest.mark.parametrize("tile", [0, 1, 4])
+@pytest.mark.parametrize("tile", [0, 1, 4, 16])
 def test_block_converged_vs_oracle(orc, tile):
     """Columns of different difficulty stop at different iterations."""
     print(f"[case] block converged tile={tile}", flush=True)
@@ -154,5 +160,7 @@ def test_block_rejects_bad_arguments():
         S.pcg_block(A, S.to_dev(rhs_block(Ah.n, 3, 1)), d_r)  # k not a power of two
     with pytest.raises(L.LapError):
         S.pcg_block(A, S.to_dev(rhs_block(Ah.n, 4, 1)), d_r, tile=3)  # not 1, 2 or 4
+    with pytest.raises(L.LapError):
+        S.pcg_block(A, S.to_dev(rhs_block(Ah.n, 4, 1)), d_r, tile=8)  # 8 needs k = 64
     with pytest.raises(L.LapError):
         S.spmm(A, S.to_dev(rhs_block(Ah.n, 128, 1)))  # k > 64
