@@ -169,6 +169,10 @@ __global__ __launch_bounds__(BS) void k_spmm64(int n, const int *__restrict__ ia
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int wave_id = (int)blockIdx.x * NW + wave, nwaves = (int)gridDim.x * NW;
   double acc = 0.0;
+  // P's own row for the p.q epilogue: the diagonal entry's gather already
+  // holds it (loaded only for a row without a stored diagonal)
+  double pown = 0.0;
+  bool have_own = false;
   auto finish = [&](int row, double sum) {
     const size_t o = (size_t)row * K + lane;
     if (MODE == 2) {
@@ -177,8 +181,9 @@ __global__ __launch_bounds__(BS) void k_spmm64(int n, const int *__restrict__ ia
       acc += r * r;
     } else {
       Q[o] = sum;
-      if (MODE == 1) acc += P[o] * sum;
+      if (MODE == 1) acc += (have_own ? pown : P[o]) * sum;
     }
+    have_own = false;
   };
   for (int base = wave_id * RU; base < n; base += nwaves * RU) {
     const int nrows = min(RU, n - base);
@@ -207,6 +212,10 @@ __global__ __launch_bounds__(BS) void k_spmm64(int n, const int *__restrict__ ia
               cur = 0.0;
               ++j;
               next = __builtin_amdgcn_readlane(ptr_l, j + 1);
+            }
+            if (MODE == 1 && __builtin_amdgcn_readlane(jl, t0 + u) == base + j) {
+              pown = v[u];
+              have_own = true;
             }
             cur += readlane_f64(al, t0 + u) * v[u];
           }
@@ -488,8 +497,8 @@ int log2i(int k) {
   return l;
 }
 
-// (ia, ja, a, n, k) -> chosen W
-std::map<std::tuple<const void *, const void *, const void *, int, int>, int> g_tile;
+// (ia, ja, a, n, k, mode) -> chosen tile
+std::map<std::tuple<const void *, const void *, const void *, int, int, int>, int> g_tile;
 
 unsigned spmm_grid() {
   const int per_cu = std::max(1, std::min(16, options().spmm_wg_per_cu));
@@ -504,8 +513,8 @@ unsigned vec_grid(size_t elems) {
 // Times every rows-per-group choice on Y = A X (three runs each, after a
 // warm-up).
 int pick_tile(int n, const int *ia, const int *ja, const double *a, int k, const double *X,
-              double *Y) {
-  auto key = std::make_tuple((const void *)ia, (const void *)ja, (const void *)a, n, k);
+              double *Y, int mode = 0, double *part = nullptr, const BlkCtl *ctl = nullptr) {
+  auto key = std::make_tuple((const void *)ia, (const void *)ja, (const void *)a, n, k, mode);
   auto it = g_tile.find(key);
   if (it != g_tile.end()) return it->second;
   int best_ru = 0;
@@ -520,9 +529,9 @@ int pick_tile(int n, const int *ia, const int *ja, const double *a, int k, const
   for (int ru = 1; ru <= 16; ru *= 2) {
     if (!tile_ok(k, ru)) continue;
     SpmmFn fn = spmm_for(k, ru);
-    fn(g, n, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
+    fn(g, n, ia, ja, a, X, Y, nullptr, part, ctl, mode);
     LAP_HIP(hipEventRecord(e0, stream()));
-    for (int r = 0; r < 3; ++r) fn(g, n, ia, ja, a, X, Y, nullptr, nullptr, nullptr, 0);
+    for (int r = 0; r < 3; ++r) fn(g, n, ia, ja, a, X, Y, nullptr, part, ctl, mode);
     LAP_HIP(hipEventRecord(e1, stream()));
     LAP_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -616,7 +625,11 @@ extern "C" int lap_pcg_block(int n, int nnz, const int *ia, const int *ja, const
   if (!R || !P || !Q || !ctl || !part || (h_hist && !hist)) return lap_last_status();
   // rows past a column's own stop stay 0
   if (hist) launch_fill((int)hist_len, 0.0, hist);
-  if (!tile) tile = pick_tile(n, ia, ja, a, k, Bm, Q);
+  if (!tile) {  // timed in the solve's own mode (q = A p with the p.q epilogue)
+    const int live = 0;
+    LAP_HIP(hipMemcpyAsync(&ctl->all_done, &live, sizeof(int), hipMemcpyHostToDevice, stream()));
+    tile = pick_tile(n, ia, ja, a, k, Bm, Q, 1, part, ctl);
+  }
   if (!tile) return LAP_ERR_HIP;
   SpmmFn spmm = spmm_for(k, tile);
   const int lk = log2i(k);

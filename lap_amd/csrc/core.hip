@@ -164,9 +164,16 @@ int lap_set_option(const char *name, int value) {
   else if (!strcmp(name, "jacobi_variant")) slot = &o.jacobi_variant;
   else if (!strcmp(name, "sptrsv_mode")) slot = &o.sptrsv_mode;
   else if (!strcmp(name, "spmm_wg_per_cu")) slot = &o.spmm_wg_per_cu;
+  else if (!strcmp(name, "sf_wg_per_cu")) slot = &o.sf_wg_per_cu;
+  else if (!strcmp(name, "sf_naps")) slot = &o.sf_naps;
   if (!slot) return -1;
   if ((!strcmp(name, "spmv_walk") && (value < 0 || value > 2)) ||
       (!strcmp(name, "vec_grid") && (value < 1 || value > kMaxPartialBlocks / 2))) {
+    set_error(LAP_ERR_ARG, "option %s: value %d out of range", name, value);
+    return *slot;
+  }
+  if ((!strcmp(name, "sf_wg_per_cu") && (value < 1 || value > 16)) ||
+      (!strcmp(name, "sf_naps") && (value < 1 || value > 64))) {
     set_error(LAP_ERR_ARG, "option %s: value %d out of range", name, value);
     return *slot;
   }

@@ -179,6 +179,8 @@ struct Options {
   int spmm_wg_per_cu = 4;  // block CG SpMM grid: workgroups per CU
   int sptrsv_mode = 0;     // triangular solves: 0 one launch per level,
                            // 1 one sync-free launch
+  int sf_wg_per_cu = 8;    // sync-free solve: workgroups per CU
+  int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
                            // 1 three kernels, x in K_C; 2 two kernels (p in the
                            // SpMV gather)

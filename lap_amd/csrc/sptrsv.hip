@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_syncfree(
     int nchunks, const int *__restrict__ chunk_ptr, const int *__restrict__ rows,
     const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
     const double *__restrict__ diag, const double *__restrict__ x, double *y, int *state,
-    const int *__restrict__ done) {
+    const int *__restrict__ done, int naps) {
   if (done && *done) return;
   const int lane = threadIdx.x & 63;
   for (;;) {
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_syncfree(
             v = __builtin_nan("");
             break;
           }
-          __builtin_amdgcn_s_sleep(2);
+          for (int z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(8);  // ~512 cycles each
           v = sf_load(y + col);
         }
         acc -= a[e] * v;
@@ -258,9 +258,12 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
   if (st && p.n > 0) {
     const unsigned gf = grid_for((size_t)p.n, 4, kBlock, 2048);
     k_sf_fill<<<gf, kBlock, 0, stream()>>>(p.n, y, st, done);
-    const unsigned g = (unsigned)std::max(1, std::min((p.nchunks + 3) / 4, 8 * device_cus()));
+    const int per_cu = std::max(1, std::min(16, options().sf_wg_per_cu));
+    const unsigned g =
+        (unsigned)std::max(1, std::min((p.nchunks + 3) / 4, per_cu * device_cus()));
     k_tri_syncfree<<<g, kBlock, 0, stream()>>>(p.nchunks, p.chunk_ptr->as<int>(), order, ia, ja,
-                                               a, diag, x, y, st, done);
+                                               a, diag, x, y, st, done,
+                                               std::max(1, options().sf_naps));
     return;
   }
   for (int l = 0; l < p.nlevels; ++l) {
