@@ -130,6 +130,7 @@ void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *
     LAP_SPMV_CASE(2)
     LAP_SPMV_CASE(3)
     LAP_SPMV_CASE(4)
+    LAP_SPMV_CASE(5)
     default:
       LAP_SPMV_CASE(0)
   }

@@ -22,7 +22,9 @@ constexpr int kSpmvTile = 2048;
 //     while the current block is summed (barriers on gfx950 wait for LDS
 //     only, not for outstanding global loads).
 //  4  as 3 with non-temporal loads of ja and a.
-constexpr int kSpmvVariants = 5;
+//  5  as 0 with non-temporal loads of ja and a (read once, so they do not
+//     push the reused x lines out of the XCD's L2).
+constexpr int kSpmvVariants = 6;
 
 template <bool NT, class T>
 __device__ __forceinline__ T ld_stream(const T *p) {
@@ -62,7 +64,7 @@ __device__ __forceinline__ void spmv_row_sums(int r0, int r1, int e0, const int 
 // Variant 0 body for row block b.  Block-uniform control flow.  Values
 // (VT) and vectors (XT) may be FP32; products and row sums are FP64.
 //  prod : __shared__ double[TILE], red : __shared__ double[BS/64].
-template <int BS, int TILE, class Epi, class VT = double, class XT = double>
+template <int BS, int TILE, class Epi, class VT = double, class XT = double, bool NT = false>
 __device__ __forceinline__ void spmv_block_rows(int b, const int *__restrict__ rowblk,
                                                 const int *__restrict__ ia,
                                                 const int *__restrict__ ja,
@@ -79,7 +81,7 @@ __device__ __forceinline__ void spmv_block_rows(int b, const int *__restrict__ r
 #pragma unroll 4
   for (int k = threadIdx.x; k < cnt; k += BS) {
     const int e = e0 + k;
-    prod[k] = (alpha * (double)a[e]) * (double)x[ja[e]];
+    prod[k] = (alpha * (double)ld_stream<NT>(a + e)) * (double)x[ld_stream<NT>(ja + e)];
   }
   __syncthreads();
   spmv_row_sums<BS>(r0, r1, e0, ia, prod, epi);
@@ -248,9 +250,10 @@ __device__ __forceinline__ void spmv_walk(BlockWalk w, const int *__restrict__ r
                                           const double *__restrict__ a,
                                           const double *__restrict__ x, double alpha,
                                           double *prod, double *red, Epi &epi) {
-  if constexpr (V == 0) {
+  if constexpr (V == 0 || V == 5) {
     for (int b = w.begin; b < w.end; b += w.step)
-      spmv_block_rows<BS, TILE>(b, rowblk, ia, ja, a, x, alpha, prod, red, epi);
+      spmv_block_rows<BS, TILE, Epi, double, double, V == 5>(b, rowblk, ia, ja, a, x, alpha, prod,
+                                                             red, epi);
   } else if constexpr (V <= 2) {
     spmv_walk_staged<BS, TILE, V == 2>(w, rowblk, ia, ja, a, x, alpha, prod, red, epi);
   } else {
