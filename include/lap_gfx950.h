@@ -329,7 +329,7 @@ double lap_pcg_fused_timed(int n, int nnz, const int *ia, const int *ja, const d
 /* Library configuration knobs (for tuning sweeps): name/value pairs.
  * "spmv_rows" (rows per SpMV block), "spmv_walk" (row-block order of the
  * persistent SpMV grid: 0 cyclic, 1 per-XCD eighths, 2 contiguous chunk per
- * workgroup), "spmv_variant" (SpMV body, 0-5), "vec_grid" (workgroups of
+ * workgroup), "spmv_variant" (SpMV body, 0-6), "vec_grid" (workgroups of
  * the vector kernels), "graph_iters" (iterations per HIP graph launch),
  * "use_graph", "spmm_wg_per_cu" (block CG SpMM workgroups per CU),
  * "sptrsv_mode" (triangular solves: 0 one launch per level, 1 one

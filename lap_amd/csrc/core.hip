@@ -189,8 +189,8 @@ int lap_set_option(const char *name, int value) {
     set_error(LAP_ERR_ARG, "jacobi_variant must be 0, 1 or 2");
     return *slot;
   }
-  if (!strcmp(name, "spmv_variant") && (value < 0 || value > 5)) {
-    set_error(LAP_ERR_ARG, "spmv_variant must be in [0, 5]");
+  if (!strcmp(name, "spmv_variant") && (value < 0 || value > 6)) {
+    set_error(LAP_ERR_ARG, "spmv_variant must be in [0, 6]");
     return *slot;
   }
   if (!strcmp(name, "spmv_rows") && (value < 1 || value > 2 * kBlock)) {

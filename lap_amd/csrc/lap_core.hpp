@@ -151,6 +151,7 @@ __device__ __forceinline__ double sum_partials(const double *partials, int cnt, 
 //                   and planes) then sit in chunks of workgroups on one XCD.
 struct BlockWalk {
   int begin, end, step;
+  int nblk;  // row blocks of the plan (the nonzero offsets follow rowblk)
 };
 __device__ __forceinline__ BlockWalk block_walk(int nblk, int mode) {
   const int g = gridDim.x, id = blockIdx.x;
@@ -159,13 +160,13 @@ __device__ __forceinline__ BlockWalk block_walk(int nblk, int mode) {
     const long long b0 = (long long)id * chunk;
     const int begin = b0 < nblk ? (int)b0 : nblk;
     const int end = begin + chunk < nblk ? begin + chunk : nblk;
-    return BlockWalk{begin, end, 1};
+    return BlockWalk{begin, end, 1, nblk};
   }
-  if (mode != 1 || g < 8) return BlockWalk{id, nblk, g};
+  if (mode != 1 || g < 8) return BlockWalk{id, nblk, g, nblk};
   const int k = id % 8, lw = id / 8;
   const int nw = g / 8 + (k < g % 8 ? 1 : 0);
   const int lo = (int)(((long long)nblk * k) / 8), hi = (int)(((long long)nblk * (k + 1)) / 8);
-  return BlockWalk{lo + lw, hi, nw};
+  return BlockWalk{lo + lw, hi, nw, nblk};
 }
 
 // -------------------------------------------------------------- options
@@ -202,7 +203,8 @@ void forget_plans(const void *p);
 struct SpmvPlan {
   int n = 0, nnz = 0;
   int nblocks = 0;        // row blocks
-  DevBufPtr rowblk;       // int[nblocks + 1]
+  DevBufPtr rowblk;       // int[2 * (nblocks + 1)]: first row of each block, then
+                          // its first nonzero (ia at that row)
   int max_row = 0;
 };
 // Finds or builds the plan for (ia, ja, n, nnz) (device arrays).

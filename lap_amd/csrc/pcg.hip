@@ -566,6 +566,7 @@ static void launch_init(unsigned g, const SpmvPlan &A, const int *ia, const int 
     LAP_CASE(3)
     LAP_CASE(4)
     LAP_CASE(5)
+    LAP_CASE(6)
     default:
       LAP_CASE(0)
   }
@@ -588,6 +589,7 @@ static void launch_spmv_pq(unsigned g, const SpmvPlan &A, const int *ia, const i
     LAP_CASE(3)
     LAP_CASE(4)
     LAP_CASE(5)
+    LAP_CASE(6)
     default:
       LAP_CASE(0)
   }

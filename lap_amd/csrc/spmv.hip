@@ -103,7 +103,11 @@ const SpmvPlan *spmv_plan(int n, int nnz, const int *ia, const int *ja) {
   }
   plan->nblocks = (int)starts.size() - 1;
   plan->max_row = max_row;
-  plan->rowblk = dev_upload(starts);
+  // row starts, then the nonzero offset of each (read by the prefetching body)
+  std::vector<int> rb(starts);
+  rb.reserve(2 * starts.size());
+  for (int r : starts) rb.push_back(n > 0 ? h_ia[r] : 0);
+  plan->rowblk = dev_upload(rb);
   SpmvPlan *raw = plan.get();
   g_plans[key] = std::move(plan);
   return raw;
@@ -131,6 +135,7 @@ void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *
     LAP_SPMV_CASE(3)
     LAP_SPMV_CASE(4)
     LAP_SPMV_CASE(5)
+    LAP_SPMV_CASE(6)
     default:
       LAP_SPMV_CASE(0)
   }

@@ -24,7 +24,11 @@ constexpr int kSpmvTile = 2048;
 //  4  as 3 with non-temporal loads of ja and a.
 //  5  as 0 with non-temporal loads of ja and a (read once, so they do not
 //     push the reused x lines out of the XCD's L2).
-constexpr int kSpmvVariants = 6;
+//  6  as 0 with a shorter chain of dependent loads per block: the block's
+//     nonzero range comes from the plan (no ia load before the ja/a loads),
+//     and the rows' pointers are staged into LDS while the products are
+//     formed (no ia load in the row-sum phase).
+constexpr int kSpmvVariants = 7;
 
 template <bool NT, class T>
 __device__ __forceinline__ T ld_stream(const T *p) {
@@ -85,6 +89,39 @@ __device__ __forceinline__ void spmv_block_rows(int b, const int *__restrict__ r
   }
   __syncthreads();
   spmv_row_sums<BS>(r0, r1, e0, ia, prod, epi);
+  __syncthreads();
+}
+
+// Variant 6 body: e0/e1 from the plan's offsets `ebl`, row pointers in LDS.
+template <int BS, int TILE, class Epi>
+__device__ __forceinline__ void spmv_block_rows_pf(int b, const int *__restrict__ rowblk,
+                                                   const int *__restrict__ ebl,
+                                                   const int *__restrict__ ia,
+                                                   const int *__restrict__ ja,
+                                                   const double *__restrict__ a,
+                                                   const double *__restrict__ x, double alpha,
+                                                   double *prod, double *red, Epi &epi) {
+  __shared__ int rp[2 * BS + 1];  // row pointers relative to e0 (blocks hold <= 2*BS rows)
+  const int r0 = rowblk[b], r1 = rowblk[b + 1];
+  const int e0 = ebl[b], e1 = ebl[b + 1];
+  const int cnt = e1 - e0;
+  if (cnt > TILE) {
+    spmv_long_row<BS>(r0, e0, e1, ja, a, x, alpha, red, epi);
+    return;
+  }
+  for (int i = threadIdx.x; i <= r1 - r0; i += BS) rp[i] = ia[r0 + i] - e0;
+#pragma unroll 4
+  for (int k = threadIdx.x; k < cnt; k += BS) {
+    const int e = e0 + k;
+    prod[k] = (alpha * a[e]) * x[ja[e]];
+  }
+  __syncthreads();
+  for (int row = r0 + (int)threadIdx.x; row < r1; row += BS) {
+    const int s0 = rp[row - r0], s1 = rp[row - r0 + 1];
+    double s = epi.init(row);
+    for (int k = s0; k < s1; ++k) s += prod[k];
+    epi.store(row, s);
+  }
   __syncthreads();
 }
 
@@ -254,6 +291,10 @@ __device__ __forceinline__ void spmv_walk(BlockWalk w, const int *__restrict__ r
     for (int b = w.begin; b < w.end; b += w.step)
       spmv_block_rows<BS, TILE, Epi, double, double, V == 5>(b, rowblk, ia, ja, a, x, alpha, prod,
                                                              red, epi);
+  } else if constexpr (V == 6) {
+    const int *ebl = rowblk + w.nblk + 1;
+    for (int b = w.begin; b < w.end; b += w.step)
+      spmv_block_rows_pf<BS, TILE>(b, rowblk, ebl, ia, ja, a, x, alpha, prod, red, epi);
   } else if constexpr (V <= 2) {
     spmv_walk_staged<BS, TILE, V == 2>(w, rowblk, ia, ja, a, x, alpha, prod, red, epi);
   } else {

@@ -62,7 +62,8 @@ def test_spmv_long_rows(orc):
 
 
 @pytest.mark.parametrize("rows,walk,variant", [(64, 1, 0), (512, 2, 0), (256, 0, 1), (1, 1, 2),
-                                               (256, 2, 3), (128, 0, 4), (256, 1, 5)])
+                                               (256, 2, 3), (128, 0, 4), (256, 1, 5), (256, 0, 6),
+                                               (512, 2, 6)])
 def test_spmv_blocking_invariant(orc, rows, walk, variant):
     """Row-block size, walk order and body variant change scheduling, never
     the bits."""
