@@ -226,6 +226,37 @@ def write_mm(path: str, A: CSR) -> None:
         np.savetxt(f, np.column_stack([r + 1, c + 1, v]), fmt=["%d", "%d", "%.17g"])
 
 
+def write_adjacency(path: str, n: int, src, dst, w=None) -> None:
+    """Adjacency of an undirected graph as Matrix Market coordinate
+    symmetric, one line per edge with row > column, no diagonal: the input of
+    the reference's Laplacian driver (read_adjacency_file, io_utils.c:80-130).
+    Pattern when w is None, else real weights (%.17g)."""
+    r = np.maximum(src, dst).astype(np.int64) + 1
+    c = np.minimum(src, dst).astype(np.int64) + 1
+    with open(path, "w") as f:
+        kind = "pattern" if w is None else "real"
+        f.write(f"%%MatrixMarket matrix coordinate {kind} symmetric\n")
+        f.write(f"{n} {n} {r.shape[0]}\n")
+        if w is None:
+            f.write("".join(f"{i} {j}\n" for i, j in zip(r, c)))
+        else:
+            f.write("".join(f"{i} {j} {v:.17g}\n" for i, j, v in zip(r, c, w)))
+
+
+def grid_edges(dims):
+    """Edges (i < j) of the 2D/3D grid graph behind grid_laplacian."""
+    dims = list(dims)
+    n = int(np.prod(dims))
+    idx = np.arange(n).reshape(dims[::-1])
+    src, dst = [], []
+    for ax in range(len(dims)):
+        a = np.take(idx, range(idx.shape[ax] - 1), axis=ax).ravel()
+        b = np.take(idx, range(1, idx.shape[ax]), axis=ax).ravel()
+        src.append(a)
+        dst.append(b)
+    return n, np.concatenate(src).astype(np.int32), np.concatenate(dst).astype(np.int32)
+
+
 def write_rhs(path: str, b: np.ndarray) -> None:
     with open(path, "w") as f:
         f.write("%%MatrixMarket matrix array real general\n")

@@ -40,3 +40,19 @@ PROBLEMS = {
     "graph_big": lambda: weighted_graph(5000, 12, 11),
 }
 CPU_PROBLEMS = ["grid2d", "grid3d", "graph"]
+
+
+def ring_chords(n, per, seed):
+    """Connected graph: a ring plus about `per` random chords per vertex
+    (edges (i < j), no duplicates)."""
+    rng = np.random.default_rng(seed)
+    s = np.arange(n)
+    d = (s + 1) % n
+    a = rng.integers(0, n, n * per)
+    b = rng.integers(0, n, n * per)
+    keep = a != b
+    src = np.concatenate([s, a[keep]])
+    dst = np.concatenate([d, b[keep]])
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    key = np.unique(lo.astype(np.int64) * n + hi)
+    return n, (key // n).astype(np.int32), (key % n).astype(np.int32)

@@ -2,20 +2,29 @@
 GS.c, it_jacobi.c, line_jacobi.c, blas.c, io_utils.c compiled with -DHIP=1 by
 oracle/build_ref.sh) linked against liblapgfx.so instead of hip_blas.o +
 rocSPARSE/rocBLAS, run on a Matrix Market input, agrees with the reference
-CPU binary on the same files: same exit reason, iteration count within +-1."""
+CPU binary on the same files: same exit reason, iteration count within +-1.
+
+The same for the reference's Laplacian driver (cg_driver.c): an adjacency
+file becomes the unweighted graph Laplacian (io_utils.c:80-130, :364-420),
+b = (-1)^i is projected off the constant vector (cg_driver.c:188-206) and the
+singular system is solved, with the projection's dot/scal/axpy running on
+the library too."""
 import os
 import re
 import subprocess
 
 import pytest
 
-from lap_amd.problems import grid_laplacian, rhs_uniform, write_mm, write_rhs
+from cases import ring_chords
+from lap_amd.problems import grid_edges, grid_laplacian, rhs_uniform, write_adjacency, write_mm, write_rhs
 
 pytestmark = pytest.mark.gpu
 
 REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref")
 DROPIN = os.path.join(REF, "lap_dropin_cg")
 CPU = os.path.join(REF, "lap_cpu_cg")
+DROPIN_LAP = os.path.join(REF, "lap_dropin_laplacian")
+CPU_LAP = os.path.join(REF, "lap_cpu_laplacian")
 
 
 def run(exe, args):
@@ -47,3 +56,29 @@ def test_reference_driver_on_gfx950_backend(tmp_path, kind):
     assert greason == creason == "CG converged"
     assert abs(gi - ci) <= 1
     assert "TRUE Norm of r" in gtext
+
+
+GRAPHS = {
+    "grid2d_64": lambda: grid_edges([64, 64]),
+    "grid3d_16": lambda: grid_edges([16, 16, 16]),
+    "ring_chords_3000": lambda: ring_chords(3000, 3, 5),
+}
+
+
+@pytest.mark.skipif(not (os.path.exists(DROPIN_LAP) and os.path.exists(CPU_LAP)),
+                    reason="Laplacian drop-in binary not built (needs the reference sources)")
+@pytest.mark.parametrize("kind", ["line_jacobi", "GS_std", "it_jacobi", "GS_it2"])
+@pytest.mark.parametrize("graph", sorted(GRAPHS))
+def test_reference_laplacian_driver_on_gfx950_backend(tmp_path, graph, kind):
+    print(f"[case] dropin laplacian graph={graph} kind={kind}", flush=True)
+    n, src, dst = GRAPHS[graph]()
+    mm = str(tmp_path / "adj.mtx")
+    write_adjacency(mm, n, src, dst)
+    k = "2" if kind in ("it_jacobi", "GS_it2") else "1"
+    args = [mm, kind, "1e-10", "25000", "1", k]
+    gi, gres, greason, gtext = run(DROPIN_LAP, args)
+    ci, cres, creason, _ = run(CPU_LAP, args)
+    print(f"[diag] dropin laplacian {graph}/{kind}: gpu it {gi} res {gres:.6e} ({greason}); "
+          f"cpu it {ci} res {cres:.6e} ({creason})", flush=True)
+    assert greason == creason == "CG converged"
+    assert abs(gi - ci) <= 1
