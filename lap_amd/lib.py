@@ -35,6 +35,14 @@ class SolveInfo(C.Structure):
                 ("refinements", C.c_int)]
 
 
+class MmInfo(C.Structure):
+    """lap_mm_info (include/lap_io.h)."""
+    _fields_ = [("n", C.c_int), ("m", C.c_int), ("nnz", C.c_int), ("nnz_csr", C.c_int),
+                ("symmetric", C.c_int), ("pattern", C.c_int)]
+
+
+IP = C.POINTER(C.c_int)
+
 # name -> (restype, argtypes)
 _SIGS = {
     "initialize_handles": (None, []),
@@ -94,13 +102,30 @@ _SIGS = {
     "lap_pcg_fused_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, PD, PD, PD, PD,
                                          C.c_int]),
     "lap_set_option": (C.c_int, [C.c_char_p, C.c_int]),
+    # input path (include/lap_io.h)
+    "lap_mm_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(VP), C.POINTER(MmInfo)]),
+    "lap_mm_coo": (C.c_int, [VP, VP, VP, VP, C.c_int]),
+    "lap_mm_free": (None, [VP]),
+    "lap_rhs_load": (C.c_int, [C.c_char_p, C.c_int, VP, C.c_int]),
+    "lap_mm_write": (C.c_int, [C.c_char_p, C.c_int, VP, VP, VP]),
+    "lap_csr_save": (C.c_int, [C.c_char_p, C.c_int, VP, VP, VP]),
+    "lap_csr_info": (C.c_int, [C.c_char_p, IP, IP]),
+    "lap_csr_load": (C.c_int, [C.c_char_p, VP, VP, VP, C.c_int]),
+    "lap_coo_to_csr": (C.c_int, [C.c_int, C.c_int, C.c_int, PI, PI, PD, C.c_int, C.c_int, PI,
+                                 PI, PD, IP]),
+    "lap_csr_split": (C.c_int, [C.c_int, PI, PI, PD, C.c_int, C.c_int, PI, PI, PD, C.c_int,
+                                PI, PI, PD, PD, IP, IP]),
+    "lap_adjacency_laplacian": (C.c_int, [C.c_int, PI, PI, PD, C.c_int, PI]),
 }
 
 # Symbols the header declares; the CPU test checks the library exports each.
 EXPORTED = sorted(list(_SIGS) + [
     "hip_ichol", "GS_std", "GS_it", "GS_it2", "it_jacobi", "line_jacobi", "dot", "axpy", "scal",
     "csr_matvec", "lower_triangular_solve", "upper_triangular_solve", "ichol", "vec_vec",
-    "vector_reciprocal", "vector_sqrt", "vec_copy", "vec_zero"])
+    "vector_reciprocal", "vector_sqrt", "vec_copy", "vec_zero",
+    # io_utils.h drop-in (include/lap_io.h)
+    "read_mm_file", "read_adjacency_file", "coo_to_csr", "read_rhs", "split",
+    "create_L_and_split"])
 
 _lib = None
 

@@ -192,16 +192,20 @@ def transpose(A: CSR) -> CSR:
 
 def from_coo(n: int, rows, cols, vals, symmetric: bool) -> CSR:
     """COO -> CSR with the semantics of coo_to_csr (io_utils.c:141-235):
-    symmetric entries are mirrored, each row is sorted by column."""
+    symmetric entries are mirrored, each row is sorted by column; equal
+    (row, column) pairs keep the fill order (entry i, then its mirror,
+    before entry i+1), as the reference's merge-sorting qsort does."""
     rows = np.asarray(rows, dtype=np.int64)
     cols = np.asarray(cols, dtype=np.int64)
     vals = np.asarray(vals, dtype=np.float64)
+    seq = 2 * np.arange(rows.shape[0], dtype=np.int64)  # the reference's fill order
     if symmetric:
         off = rows != cols
-        rows, cols, vals = (np.concatenate([rows, cols[off]]),
-                            np.concatenate([cols, rows[off]]),
-                            np.concatenate([vals, vals[off]]))
-    order = np.lexsort((cols, rows))
+        rows, cols, vals, seq = (np.concatenate([rows, cols[off]]),
+                                 np.concatenate([cols, rows[off]]),
+                                 np.concatenate([vals, vals[off]]),
+                                 np.concatenate([seq, seq[off] + 1]))
+    order = np.lexsort((seq, cols, rows))
     rows, cols, vals = rows[order], cols[order], vals[order]
     ia = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(np.bincount(rows, minlength=n), out=ia[1:])

@@ -401,3 +401,189 @@ void orc_edge_grad(int m, const int *src, const int *dst, const double *lam, con
                    double *g) {
   for (int e = 0; e < m; ++e) g[e] = -((lam[src[e]] - lam[dst[e]]) * (x[src[e]] - x[dst[e]]));
 }
+
+/* ------------------------------------------------------------ input path */
+
+long orc_mm_read(const char *path, int adjacency, int *n, int *m, int *symmetric, int *rows,
+                 int *cols, double *vals) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char line[512];
+  if (!fgets(line, sizeof line, f)) {
+    fclose(f);
+    return -1;
+  }
+  const int pattern = strstr(line, "pattern") != NULL;
+  const int sym = adjacency || strstr(line, "symmetric") != NULL;
+  while (line[0] == '%')
+    if (!fgets(line, sizeof line, f)) {
+      fclose(f);
+      return -1;
+    }
+  long nn, mm, nz;
+  if (sscanf(line, "%ld %ld %ld", &nn, &mm, &nz) != 3) {
+    fclose(f);
+    return -1;
+  }
+  long k = 0;
+  while (k < nz && fgets(line, sizeof line, f)) {
+    char *p = line;
+    while (*p == ' ' || *p == '\t') ++p;
+    if (*p == '%' || *p == '\n' || *p == '\r' || *p == 0) continue;
+    char *e;
+    const long r = strtol(p, &e, 10);
+    const long c = strtol(e, &e, 10);
+    double v = 1.0;
+    if (!pattern) v = strtod(e, &e);
+    if (rows) {
+      rows[k] = (int)(r - 1);
+      cols[k] = (int)(c - 1);
+      vals[k] = v;
+    }
+    ++k;
+  }
+  fclose(f);
+  if (k != nz) return -1;
+  if (adjacency) {
+    for (long j = 0; j < nn; ++j, ++k)
+      if (rows) {
+        rows[k] = (int)j;
+        cols[k] = (int)j;
+        vals[k] = 1.0;
+      }
+  }
+  *n = (int)nn;
+  *m = (int)mm;
+  *symmetric = sym;
+  return k;
+}
+
+/* stable bottom-up merge sort of (key, val) by key */
+static void merge_sort_row(int len, int *key, double *val, int *tk, double *tv) {
+  for (int w = 1; w < len; w *= 2) {
+    for (int lo = 0; lo < len; lo += 2 * w) {
+      const int mid = lo + w < len ? lo + w : len, hi = lo + 2 * w < len ? lo + 2 * w : len;
+      int i = lo, j = mid, o = lo;
+      while (i < mid && j < hi) {
+        if (key[j] < key[i]) {
+          tk[o] = key[j];
+          tv[o++] = val[j++];
+        } else {
+          tk[o] = key[i];
+          tv[o++] = val[i++];
+        }
+      }
+      while (i < mid) {
+        tk[o] = key[i];
+        tv[o++] = val[i++];
+      }
+      while (j < hi) {
+        tk[o] = key[j];
+        tv[o++] = val[j++];
+      }
+    }
+    memcpy(key, tk, sizeof(int) * (size_t)len);
+    memcpy(val, tv, sizeof(double) * (size_t)len);
+  }
+}
+
+long orc_coo_to_csr(int n, long nnz, const int *rows, const int *cols, const double *vals,
+                    int symmetric, int *ia, int *ja, double *a) {
+  long *cnt = calloc((size_t)n + 1, sizeof(long));
+  for (long i = 0; i < nnz; ++i) {
+    cnt[rows[i]]++;
+    if (symmetric && rows[i] != cols[i]) cnt[cols[i]]++;
+  }
+  ia[0] = 0;
+  for (int i = 0; i < n; ++i) ia[i + 1] = ia[i] + (int)cnt[i];
+  const long total = ia[n];
+  if (ja) {
+    for (int i = 0; i < n; ++i) cnt[i] = ia[i];
+    for (long i = 0; i < nnz; ++i) {
+      const long p = cnt[rows[i]]++;
+      ja[p] = cols[i];
+      a[p] = vals[i];
+      if (symmetric && rows[i] != cols[i]) {
+        const long q = cnt[cols[i]]++;
+        ja[q] = rows[i];
+        a[q] = vals[i];
+      }
+    }
+    int maxlen = 1;
+    for (int i = 0; i < n; ++i)
+      if (ia[i + 1] - ia[i] > maxlen) maxlen = ia[i + 1] - ia[i];
+    int *tk = malloc(sizeof(int) * (size_t)maxlen);
+    double *tv = malloc(sizeof(double) * (size_t)maxlen);
+    for (int i = 0; i < n; ++i) merge_sort_row(ia[i + 1] - ia[i], ja + ia[i], a + ia[i], tk, tv);
+    free(tk);
+    free(tv);
+  }
+  free(cnt);
+  return total;
+}
+
+long orc_split(int n, const int *ia, const int *ja, const double *a, int augmented, int *lia,
+               int *lja, double *la, int *uia, int *uja, double *ua, double *d, long *nnz_u) {
+  long nl = 0, nu = 0;
+  for (int i = 0; i < n; ++i) {
+    lia[i] = (int)nl;
+    uia[i] = (int)nu;
+    double dv = 0.0;
+    long pu = nu + augmented;
+    for (int p = ia[i]; p < ia[i + 1]; ++p) {
+      const int c = ja[p];
+      if (c < i) {
+        if (lja) {
+          lja[nl] = c;
+          la[nl] = a[p];
+        }
+        ++nl;
+      } else if (c > i) {
+        if (uja) {
+          uja[pu] = c;
+          ua[pu] = a[p];
+        }
+        ++pu;
+      } else {
+        dv = a[p];
+      }
+    }
+    if (augmented) {
+      if (lja) {
+        lja[nl] = i;
+        la[nl] = dv;
+        uja[nu] = i;
+        ua[nu] = dv;
+      }
+      ++nl;
+    }
+    nu = pu;
+    if (d) d[i] = dv;
+  }
+  lia[n] = (int)nl;
+  uia[n] = (int)nu;
+  *nnz_u = nu;
+  return nl;
+}
+
+void orc_adjacency_laplacian(int n, const int *ia, const int *ja, double *a, int weighted,
+                             int *deg) {
+  int *dd = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    dd[i] = -1;
+    for (int p = ia[i]; p < ia[i + 1]; ++p) dd[i] += a[p]; /* int += double truncates */
+  }
+  for (int i = 0; i < n; ++i) {
+    const double ds = weighted ? 1.0 / sqrt(dd[i]) : 0.0;
+    for (int p = ia[i]; p < ia[i + 1]; ++p) {
+      if (ja[p] == i)
+        a[p] = weighted ? 1.0 : (double)dd[i];
+      else if (!weighted)
+        a[p] = (-1.0) * a[p];
+      else
+        a[p] = (-1.0) * a[p] * ds * (1.0 / sqrt(dd[ja[p]]));
+    }
+  }
+  if (deg) memcpy(deg, dd, sizeof(int) * (size_t)n);
+  free(dd);
+}

@@ -112,6 +112,32 @@ void orc_graph_values(int n, int m, const int *ia, const int *ja, const int *epo
 void orc_edge_grad(int m, const int *src, const int *dst, const double *lam, const double *x,
                    double *g);
 
+/* ---- input path (io_utils.c) ------------------------------------------ */
+/* Matrix Market coordinate reader, serial (io_utils.c:15-74, adjacency mode
+ * :80-130): banner decides pattern / symmetric, 1-based entries to 0-based
+ * COO in file order, values by strtod.  Returns the COO count (with the n
+ * appended diagonal slots in adjacency mode) and fills n, m, symmetric;
+ * with rows == NULL only counts.  Returns -1 on a malformed file. */
+long orc_mm_read(const char *path, int adjacency, int *n, int *m, int *symmetric, int *rows,
+                 int *cols, double *vals);
+/* COO -> CSR (io_utils.c:141-235): rows counted, entries (and mirrors when
+ * symmetric) filled in COO order, each row then stably merge-sorted by
+ * column.  Returns the CSR entry count; with ja == NULL only counts (ia is
+ * still filled). */
+long orc_coo_to_csr(int n, long nnz, const int *rows, const int *cols, const double *vals,
+                    int symmetric, int *ia, int *ja, double *a);
+/* A = L + D + U (io_utils.c:268-363); augmented = 1 adds the diagonal last
+ * in L and first in U (cg_driver2.c:95-144).  d (may be NULL) gets the
+ * row's last diagonal entry or 0.  Returns nnz(L) and sets *nnz_u; with lja
+ * == NULL only counts. */
+long orc_split(int n, const int *ia, const int *ja, const double *a, int augmented, int *lia,
+               int *lja, double *la, int *uia, int *uja, double *ua, double *d, long *nnz_u);
+/* In place: adjacency (with diagonal slots) -> Laplacian (io_utils.c:
+ * 383-425): int degree accumulated in CSR order from -1, unweighted or
+ * normalised.  deg (may be NULL) receives the degrees. */
+void orc_adjacency_laplacian(int n, const int *ia, const int *ja, double *a, int weighted,
+                             int *deg);
+
 #ifdef __cplusplus
 }
 #endif

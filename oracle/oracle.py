@@ -87,6 +87,14 @@ class Oracle:
                                  C.c_int, PI, PI, PD, C.c_int, C.c_int]
         L.orc_graph_values.argtypes = [C.c_int, C.c_int, PI, PI, PI, PI, PD, C.c_double, PD]
         L.orc_edge_grad.argtypes = [C.c_int, PI, PI, PD, PD, PD]
+        L.orc_mm_read.restype = C.c_long
+        L.orc_mm_read.argtypes = [C.c_char_p, C.c_int, PI, PI, PI, PI, PI, PD]
+        L.orc_coo_to_csr.restype = C.c_long
+        L.orc_coo_to_csr.argtypes = [C.c_int, C.c_long, PI, PI, PD, C.c_int, PI, PI, PD]
+        L.orc_split.restype = C.c_long
+        L.orc_split.argtypes = [C.c_int, PI, PI, PD, C.c_int, PI, PI, PD, PI, PI, PD, PD,
+                                C.POINTER(C.c_long)]
+        L.orc_adjacency_laplacian.argtypes = [C.c_int, PI, PI, PD, C.c_int, PI]
 
     # -- kernels --------------------------------------------------------
     def dot(self, v, w):
@@ -108,6 +116,54 @@ class Oracle:
         g = np.zeros(src.shape[0])
         self.lib.orc_edge_grad(src.shape[0], ip(src), ip(dst), dp(lam), dp(x), dp(g))
         return g
+
+    # -- input path -------------------------------------------------------
+    def mm_read(self, path: str, adjacency: bool = False):
+        """(n, m, symmetric, rows, cols, vals): 0-based COO in file order."""
+        n, m, sym = C.c_int(0), C.c_int(0), C.c_int(0)
+        k = self.lib.orc_mm_read(path.encode(), int(adjacency), C.byref(n), C.byref(m),
+                                 C.byref(sym), None, None, None)
+        if k < 0:
+            raise ValueError(f"oracle cannot parse {path}")
+        rows, cols = np.zeros(k, np.int32), np.zeros(k, np.int32)
+        vals = np.zeros(k)
+        self.lib.orc_mm_read(path.encode(), int(adjacency), C.byref(n), C.byref(m),
+                             C.byref(sym), ip(rows), ip(cols), dp(vals))
+        return n.value, m.value, bool(sym.value), rows, cols, vals
+
+    def coo_to_csr(self, n, rows, cols, vals, symmetric):
+        from lap_amd.problems import CSR
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        vals = np.ascontiguousarray(vals, np.float64)
+        ia = np.zeros(n + 1, np.int32)
+        tot = self.lib.orc_coo_to_csr(n, rows.shape[0], ip(rows), ip(cols), dp(vals),
+                                      int(symmetric), ip(ia), None, None)
+        ja, a = np.zeros(tot, np.int32), np.zeros(tot)
+        self.lib.orc_coo_to_csr(n, rows.shape[0], ip(rows), ip(cols), dp(vals), int(symmetric),
+                                ip(ia), ip(ja), dp(a))
+        return CSR(n, ia, ja, a)
+
+    def split(self, A, augmented: bool = False):
+        """(L, d, U) of A = L + D + U; augmented adds the diagonal to both."""
+        from lap_amd.problems import CSR
+        lia, uia = np.zeros(A.n + 1, np.int32), np.zeros(A.n + 1, np.int32)
+        nu = C.c_long(0)
+        nl = self.lib.orc_split(A.n, ip(A.ia), ip(A.ja), dp(A.a), int(augmented), ip(lia), None,
+                                None, ip(uia), None, None, None, C.byref(nu))
+        lja, la = np.zeros(nl, np.int32), np.zeros(nl)
+        uja, ua = np.zeros(nu.value, np.int32), np.zeros(nu.value)
+        d = np.zeros(A.n)
+        self.lib.orc_split(A.n, ip(A.ia), ip(A.ja), dp(A.a), int(augmented), ip(lia), ip(lja),
+                           dp(la), ip(uia), ip(uja), dp(ua), dp(d), C.byref(nu))
+        return CSR(A.n, lia, lja, la), d, CSR(A.n, uia, uja, ua)
+
+    def adjacency_laplacian(self, A, weighted: bool = False):
+        """(values of the Laplacian on A's pattern, int degrees)."""
+        a = A.a.copy()
+        deg = np.zeros(A.n, np.int32)
+        self.lib.orc_adjacency_laplacian(A.n, ip(A.ia), ip(A.ja), dp(a), int(weighted), ip(deg))
+        return a, deg
 
     def lower_solve(self, T, x, diag=None):
         y = np.zeros(T.n)
@@ -231,6 +287,98 @@ class Reference:
             self.lib.cg(A.n, float(A.nnz), ip(A.ia), ip(A.ja), dp(A.a), dp(x), dp(bb), tol,
                         C.byref(pd), maxit, C.byref(it), C.byref(flag), dp(hist))
         return x, it.value, flag.value, hist[: it.value + 1].copy()
+
+
+class MMatrix(C.Structure):
+    """Layout of the reference's mmatrix (io_utils.h:27-40)."""
+    _fields_ = [("coo_rows", PI), ("coo_cols", PI), ("coo_vals", PD),
+                ("csr_ia", PI), ("csr_ja", PI), ("csr_vals", PD),
+                ("n", C.c_int), ("m", C.c_int), ("nnz", C.c_int), ("nnz_unpacked", C.c_int),
+                ("symmetric", C.c_int)]
+
+
+class IoUtils:
+    """Calls the io_utils.h functions (read_mm_file, read_adjacency_file,
+    coo_to_csr, read_rhs, split, create_L_and_split) of a shared library that
+    defines them -- the reference's compiled CPU path or a library that
+    replaces io_utils.c -- and copies every result into numpy arrays."""
+
+    def __init__(self, lib: C.CDLL):
+        self.lib = lib
+        P = C.POINTER(MMatrix)
+        for nm in ("read_mm_file", "read_adjacency_file"):
+            getattr(lib, nm).argtypes = [C.c_char_p, P]
+            getattr(lib, nm).restype = None
+        lib.coo_to_csr.argtypes = [P]
+        lib.coo_to_csr.restype = None
+        lib.read_rhs.argtypes = [C.c_char_p, PD]
+        lib.read_rhs.restype = None
+        lib.split.argtypes = [P, P, P, P]
+        lib.split.restype = None
+        lib.create_L_and_split.argtypes = [P, P, P, P, C.c_int]
+        lib.create_L_and_split.restype = None
+        self.libc = C.CDLL(None)
+        self.libc.free.argtypes = [C.c_void_p]
+
+    @staticmethod
+    def _arr(p, count, dtype):
+        if count <= 0 or not p:
+            return np.zeros(0, dtype)
+        return np.ctypeslib.as_array(p, shape=(count,)).astype(dtype, copy=True)
+
+    def _free(self, *ptrs):
+        for p in ptrs:
+            if p:
+                self.libc.free(C.cast(p, C.c_void_p))
+
+    def read(self, path: str, adjacency: bool = False):
+        """COO after read_mm_file / read_adjacency_file:
+        (n, m, nnz, symmetric, rows, cols, vals)."""
+        A = MMatrix()
+        fn = self.lib.read_adjacency_file if adjacency else self.lib.read_mm_file
+        with _silence_stdout():
+            fn(path.encode(), C.byref(A))
+        out = (A.n, A.m, A.nnz, A.symmetric, self._arr(A.coo_rows, A.nnz, np.int32),
+               self._arr(A.coo_cols, A.nnz, np.int32), self._arr(A.coo_vals, A.nnz, np.float64))
+        self._free(A.coo_rows, A.coo_cols, A.coo_vals)
+        return out
+
+    def _csr(self, M, n, nnz):
+        from lap_amd.problems import CSR
+        ia = self._arr(M.csr_ia, n + 1, np.int32)
+        return CSR(n, ia, self._arr(M.csr_ja, int(ia[-1]), np.int32),
+                   self._arr(M.csr_vals, int(ia[-1]), np.float64))
+
+    def load(self, path: str, adjacency: bool = False, weighted: int | None = None):
+        """read -> coo_to_csr -> split (or create_L_and_split when weighted is
+        not None).  Returns dict(A, L, U, D, fields) with every array copied."""
+        A, L, U, D = MMatrix(), MMatrix(), MMatrix(), MMatrix()
+        fn = self.lib.read_adjacency_file if adjacency else self.lib.read_mm_file
+        with _silence_stdout():
+            fn(path.encode(), C.byref(A))
+            self.lib.coo_to_csr(C.byref(A))
+            if weighted is None:
+                self.lib.split(C.byref(A), C.byref(L), C.byref(U), C.byref(D))
+            else:
+                self.lib.create_L_and_split(C.byref(A), C.byref(L), C.byref(U), C.byref(D),
+                                            int(weighted))
+        n = A.n
+        res = dict(A=self._csr(A, n, A.nnz_unpacked), L=self._csr(L, n, L.nnz),
+                   U=self._csr(U, n, U.nnz),
+                   D_vals=self._arr(D.csr_vals, n, np.float64),
+                   D_ia=self._arr(D.csr_ia, n + 1, np.int32),
+                   D_ja=self._arr(D.csr_ja, n, np.int32),
+                   fields=dict(n=A.n, m=A.m, nnz=A.nnz, nnz_unpacked=A.nnz_unpacked,
+                               symmetric=A.symmetric, L_nnz=L.nnz, U_nnz=U.nnz, D_nnz=D.nnz))
+        for M in (A, L, U, D):
+            self._free(M.csr_ia, M.csr_ja, M.csr_vals)
+        return res
+
+    def read_rhs(self, path: str, n: int):
+        b = np.zeros(n)
+        with _silence_stdout():
+            self.lib.read_rhs(path.encode(), dp(b))
+        return b
 
 
 class _silence_stdout:

@@ -1,0 +1,126 @@
+"""Matrix Market test files for the input-path tests.
+
+Each writer produces a file whose parse the reference's read_mm_file /
+read_adjacency_file defines (io_utils.c:15-130): a banner, optional comment
+lines, the size line, then exactly nnz entry lines.  The cases cover the
+formats and the quirks that matter for parity: symmetric and general,
+pattern and real, duplicate (row, column) pairs, diagonal entries, unsorted
+entry order, CRLF line ends, tabs, exponent / sign / hexadecimal value
+syntax, long rows (several thousand entries), non-integer adjacency weights
+(the int degree accumulator truncates) and an adjacency file's missing
+diagonal.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from lap_amd.problems import grid_laplacian, lower_coo
+
+
+def _write(path, banner, n, m, rows, cols, vals=None, comments=(), crlf=False, fmt="%.17g",
+           sep=" "):
+    nl = "\r\n" if crlf else "\n"
+    lines = [banner] + ["%" + c for c in comments] + [f"{n} {m} {len(rows)}"]
+    if vals is None:
+        lines += [f"{r + 1}{sep}{c + 1}" for r, c in zip(rows, cols)]
+    else:
+        lines += [f"{r + 1}{sep}{c + 1}{sep}" + (v if isinstance(v, str) else fmt % v)
+                  for r, c, v in zip(rows, cols, vals)]
+    with open(path, "w", newline="") as f:
+        f.write(nl.join(lines) + nl)
+    return path
+
+
+def grid_symmetric(d, dims=(6, 5, 4), shift=1e-3, shuffle=True):
+    """Lower triangle of a shifted grid Laplacian, entries shuffled."""
+    A = grid_laplacian(list(dims), shift)
+    r, c, v = lower_coo(A)
+    if shuffle:
+        p = np.random.default_rng(1).permutation(r.shape[0])
+        r, c, v = r[p], c[p], v[p]
+    return _write(os.path.join(d, "grid_sym.mtx"),
+                  "%%MatrixMarket matrix coordinate real symmetric", A.n, A.n, r, c, v,
+                  comments=["generated grid", " second comment"])
+
+
+def general_with_duplicates(d, n=300, seed=3):
+    """General real matrix, random order, duplicate pairs, varied value
+    syntax, a few rows with thousands of entries, CRLF line ends, tabs."""
+    rng = np.random.default_rng(seed)
+    k = 6 * n
+    rows = rng.integers(0, n, k)
+    cols = rng.integers(0, n, k)
+    dup = rng.integers(0, k, 200)
+    rows = np.concatenate([rows, rows[dup], np.full(5000, 7), np.full(4500, 11)])
+    cols = np.concatenate([cols, cols[dup], rng.integers(0, n, 5000), rng.integers(0, n, 4500)])
+    vals = list(rng.standard_normal(rows.shape[0]))
+    special = ["1e-3", "-0.5", "3", "+2.5E+01", "0x1.8p1", "-7.25e-300", "1.0000000000000002"]
+    for i, s in enumerate(special):
+        vals[37 * i + 1] = s
+    return _write(os.path.join(d, "general_dup.mtx"),
+                  "%%MatrixMarket matrix coordinate real general", n, n, rows, cols, vals,
+                  crlf=True, sep="\t")
+
+
+def long_rows_symmetric(d, n=20000, seed=4):
+    """Symmetric file whose mirrored entries make two rows of ~9000 and
+    ~20000 entries (the multi-run merge path), plus a random sparse part."""
+    rng = np.random.default_rng(seed)
+    rows = list(range(n))  # diagonal
+    cols = list(range(n))
+    for hub, count in ((3, 9000), (5, n - 6)):
+        others = rng.choice(np.arange(hub + 1, n), size=count, replace=False)
+        rows += list(others)
+        cols += [hub] * count
+    extra_r = rng.integers(1, n, 3 * n)
+    extra_c = (extra_r * rng.random(3 * n)).astype(np.int64)
+    rows += list(extra_r)
+    cols += list(extra_c)
+    vals = rng.standard_normal(len(rows))
+    return _write(os.path.join(d, "long_rows.mtx"),
+                  "%%MatrixMarket matrix coordinate real symmetric", n, n, np.array(rows),
+                  np.array(cols), vals)
+
+
+def adjacency_pattern(d, n=500, seed=5):
+    """Pattern adjacency of a random graph (row > column, no diagonal)."""
+    rng = np.random.default_rng(seed)
+    edges = set()
+    while len(edges) < 4 * n:
+        i, j = (int(v) for v in rng.integers(0, n, 2))
+        if i != j:
+            edges.add((max(i, j), min(i, j)))
+    e = np.array(sorted(edges, key=lambda t: rng.random()))
+    return _write(os.path.join(d, "adj_pattern.mtx"),
+                  "%%MatrixMarket matrix coordinate pattern symmetric", n, n, e[:, 0], e[:, 1])
+
+
+def adjacency_weighted(d, n=400, seed=6):
+    """Real weights in (0.1, 2.6): the int degree accumulator truncates at
+    every step, so degrees differ from the rounded weight sums."""
+    rng = np.random.default_rng(seed)
+    edges = set()
+    while len(edges) < 3 * n:
+        i, j = (int(v) for v in rng.integers(0, n, 2))
+        if i != j:
+            edges.add((max(i, j), min(i, j)))
+    e = np.array(sorted(edges))
+    w = 0.1 + 2.5 * rng.random(e.shape[0])
+    return _write(os.path.join(d, "adj_weighted.mtx"),
+                  "%%MatrixMarket matrix coordinate real symmetric", n, n, e[:, 0], e[:, 1], w)
+
+
+def rhs_file(d, n, seed=7):
+    b = np.random.default_rng(seed).standard_normal(n)
+    path = os.path.join(d, "rhs.mtx")
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix array real general\n% comment\n")
+        f.write(f"{n} 1\n")
+        f.write("".join(f"{v:.17g}\n" for v in b))
+    return path, b
+
+
+ALL_MATRICES = [grid_symmetric, general_with_duplicates, long_rows_symmetric]
+ALL_ADJACENCY = [adjacency_pattern, adjacency_weighted]

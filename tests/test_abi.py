@@ -1,5 +1,5 @@
 """The C ABI library loads and exports every symbol include/lap_gfx950.h
-declares (no device calls: runs without a GPU)."""
+and include/lap_io.h declare (no device calls: runs without a GPU)."""
 import ctypes
 import os
 import re
@@ -8,12 +8,12 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "lap_gfx950.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("lap_gfx950.h", "lap_io.h")]
 LIB = os.path.join(ROOT, "lap_amd", "liblapgfx.so")
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "\n".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:[\w\s\*]+?)\b(\w+)\s*\(", text, flags=re.M)
     return sorted({n for n in names if n not in ("if", "while", "for", "return", "sizeof")})
@@ -22,7 +22,8 @@ def declared_functions():
 def test_header_parses():
     names = declared_functions()
     for must in ("hip_csr_matvec", "hip_dot", "initialize_ichol", "hip_ichol", "cg",
-                 "mallocForDevice", "memcpyDevice", "freeDevice", "lap_pcg"):
+                 "mallocForDevice", "memcpyDevice", "freeDevice", "lap_pcg", "lap_mm_load",
+                 "lap_coo_to_csr", "read_mm_file", "coo_to_csr", "create_L_and_split"):
         assert must in names
 
 
