@@ -244,6 +244,123 @@ def bench_adjoint(torch, np, n: int = 1_000_000, tol: float = 1e-10) -> dict:
     }
 
 
+def _warm(path: str) -> None:
+    """Reads a file once so every timed reader finds it in the page cache."""
+    with open(path, "rb") as f:
+        while f.read(1 << 26):
+            pass
+
+
+def bench_input(S, torch, np, A, Ah, size: int) -> dict:
+    """Input path on the size^3 matrix.  The matrix is written as a Matrix
+    Market file (lower triangle, %.17g) and read back: to a device CSR
+    (lap_mm_load on all host cores + lap_coo_to_csr on the device), split into
+    the GS_std triangles on the device (lap_csr_split), through the library's
+    io_utils.h definitions on the host (read_mm_file + coo_to_csr + split),
+    and as a binary CSR streamed to the device.  Every file is read once
+    before its timing, so all readers see the page cache."""
+    import shutil
+    import tempfile
+
+    from lap_amd import io as IO
+    d = tempfile.mkdtemp(prefix="lap_io_")
+    try:
+        path = os.path.join(d, f"grid{size}.mtx")
+        t0 = time.perf_counter()
+        IO.write_mm(path, Ah)
+        t_write = time.perf_counter() - t0
+        fbytes = os.path.getsize(path)
+        _warm(path)
+        D, info, t = IO.load_mm(path, times=True)
+        same = bool(torch.equal(D.ia, A.ia) and torch.equal(D.ja, A.ja)
+                    and torch.equal(D.a, A.a))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Lt, dv, Ut = IO.split(D, augmented=True)
+        torch.cuda.synchronize()
+        t_split = time.perf_counter() - t0
+        del Lt, Ut, dv
+        t0 = time.perf_counter()
+        host = IO.host_load(path)
+        t_host = time.perf_counter() - t0
+        host_same = bool(np.array_equal(host[0].ja, Ah.ja) and np.array_equal(host[0].a, Ah.a))
+        del host
+        bpath = os.path.join(d, f"grid{size}.lapcsr")
+        IO.save_csr(bpath, D)
+        _warm(bpath)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E = IO.load_csr(bpath)
+        torch.cuda.synchronize()
+        t_bin = time.perf_counter() - t0
+        bin_same = bool(torch.equal(E.ja, A.ja) and torch.equal(E.a, A.a))
+        bbytes = os.path.getsize(bpath)
+        del D, E
+        return {
+            "workload": f"{size}^3 Laplacian as Matrix Market (lower triangle, {info.nnz} "
+                        f"entries, {fbytes / 1e9:.2f} GB) and as binary CSR "
+                        f"({bbytes / 1e9:.2f} GB), page cache warm",
+            "value": t.total_s, "unit": "s",
+            "device_csr_s": {"parse": t.parse_s, "upload_coo": t.upload_s,
+                             "coo_to_csr": t.convert_s},
+            "parse_gbs": fbytes / t.parse_s / 1e9,
+            "split_augmented_device_s": t_split,
+            "host_io_utils_s": t_host,
+            "binary_csr_to_device_s": t_bin,
+            "binary_gbs": bbytes / t_bin / 1e9,
+            "write_mm_s": t_write,
+            "identical_to_generated": same and host_same and bin_same,
+            "host_threads": os.cpu_count(),
+        }
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def cpu_input_baseline(np, size: int = 128) -> dict:
+    """The reference's read_mm_file + coo_to_csr + split (io_utils.c compiled
+    into oracle/_ref, one thread) on a size^3 Matrix Market file, beside the
+    library's device path and io_utils.h definitions on the same file."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    from lap_amd import io as IO
+    from lap_amd.problems import grid_laplacian
+    if not os.path.exists(oracle.REF_CPU):
+        return {"error": "reference CPU path not built (oracle/_ref)"}
+    Ah = grid_laplacian([size] * 3, 1e-8)
+    d = tempfile.mkdtemp(prefix="lap_io_ref_")
+    try:
+        path = os.path.join(d, f"grid{size}.mtx")
+        IO.write_mm(path, Ah)
+        _warm(path)
+        ref = oracle.IoUtils(oracle.Reference(oracle.REF_CPU).lib)
+        t0 = time.perf_counter()
+        R = ref.load(path)
+        t_ref = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        D, _ = IO.load_mm(path)
+        IO.split(D, augmented=True)
+        torch.cuda.synchronize()
+        t_dev = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        IO.host_load(path)
+        t_host = time.perf_counter() - t0
+        same = bool(np.array_equal(R["A"].ja, Ah.ja) and np.array_equal(R["A"].a, Ah.a))
+        return {"value": t_ref, "unit": "s", "cores": 1, "kind": "reference",
+                "sample": f"read_mm_file + coo_to_csr + split on the {size}^3 file "
+                          f"({os.path.getsize(path) / 1e9:.2f} GB, page cache warm)",
+                "library_device_s": t_dev, "library_host_io_utils_s": t_host,
+                "speedup_device": t_ref / t_dev, "speedup_host": t_ref / t_host,
+                "reference_matches": same}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def pmc_probe(S, torch, size: int, reps: int) -> int:
     """Launches for rocprofv3 --pmc: 4 elementwise copies (dst = src * 1) of a
     1 GiB float64 buffer (read 1 GiB, write 1 GiB each: the calibration of
@@ -283,7 +400,8 @@ def main() -> int:
                     help="Jacobi iteration: 0 three launches, x beside r; 1 three launches, "
                          "x beside p; 2 two launches, p update in the SpMV gather")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the multi-RHS, batched and adjoint measurements")
+                    help="skip the preconditioner, multi-RHS, batched, adjoint and input-path "
+                         "measurements")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the kernels the HBM-traffic counters are read on: a 1 GiB "
                          "device copy (calibration) and the fused SpMV, then exit")
@@ -414,7 +532,8 @@ def main() -> int:
                           lambda: bench_preconditioners(S, torch, A, Ah, b, args.tol)),
                          ("block", lambda: bench_block(S, torch, np)),
                          ("batched", lambda: bench_batched(S, torch, np)),
-                         ("adjoint", lambda: bench_adjoint(torch, np))):
+                         ("adjoint", lambda: bench_adjoint(torch, np)),
+                         ("input", lambda: bench_input(S, torch, np, A, Ah, args.size))):
             t0 = time.perf_counter()
             try:
                 out[name] = fn()
@@ -427,6 +546,14 @@ def main() -> int:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(Ah, b_h, args.prec, args.tol, its, args.cpu_iters)
         log(f"cpu baseline in {time.perf_counter() - t0:.1f}s: {out['cpu_baseline']['value']:.1f} s")
+        if not args.no_extras:
+            t0 = time.perf_counter()
+            try:
+                out["input"]["cpu_baseline"] = cpu_input_baseline(np)
+            except Exception as exc:  # reported, never hides the main line
+                out["input"]["cpu_baseline"] = {"error": f"{type(exc).__name__}: {exc}"}
+            log(f"input cpu baseline in {time.perf_counter() - t0:.1f}s: "
+                f"{json.dumps(out['input']['cpu_baseline'])}")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -114,10 +114,63 @@ inline const char *parse_int(const char *s, const char *end, long long &v) {
   return s;
 }
 
-// A value after blanks, converted by strtod (sscanf "%lf").  The token is
-// copied into a terminated buffer: the mapping has no terminator.
+// Clinger's fast path: a decimal [+-]digits[.digits][(e|E)[+-]digits] whose
+// significant digits form an integer m <= 2^53 and whose decimal exponent e
+// lies in [-22, 22] is m * 10^e or m / 10^-e with both operands exact in
+// double, so one IEEE operation gives the correctly rounded value, which is
+// what strtod returns.  Returns nullptr (caller falls back to strtod) for
+// anything else, including hexadecimal, inf and nan.
+inline const char *parse_decimal_fast(const char *s, const char *end, double &v) {
+  static constexpr double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                                        1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                                        1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+  const char *p = s;
+  bool neg = false;
+  if (p < end && (*p == '-' || *p == '+')) neg = *p++ == '-';
+  uint64_t m = 0;
+  int sig = 0, e10 = 0, ndig = 0;
+  for (; p < end && *p >= '0' && *p <= '9'; ++p, ++ndig) {
+    if (m == 0 && *p == '0') continue;  // leading zeros are not significant
+    if (++sig > 19) return nullptr;
+    m = m * 10 + (uint64_t)(*p - '0');
+  }
+  if (p < end && *p == '.') {
+    for (++p; p < end && *p >= '0' && *p <= '9'; ++p, ++ndig) {
+      --e10;
+      if (m == 0 && *p == '0') continue;
+      if (++sig > 19) return nullptr;
+      m = m * 10 + (uint64_t)(*p - '0');
+    }
+  }
+  if (ndig == 0) return nullptr;
+  if (p < end && (*p == 'e' || *p == 'E')) {
+    const char *q = p + 1;
+    bool eneg = false;
+    if (q < end && (*q == '-' || *q == '+')) eneg = *q++ == '-';
+    if (q >= end || *q < '0' || *q > '9') return nullptr;
+    int x = 0;
+    for (; q < end && *q >= '0' && *q <= '9'; ++q)
+      if (x < 10000) x = x * 10 + (*q - '0');
+    e10 += eneg ? -x : x;
+    p = q;
+  }
+  if (p < end && !blank(*p) && *p != '\n') return nullptr;  // token continues
+  if (m > (1ull << 53)) return nullptr;
+  double r = (double)m;
+  if (m != 0) {
+    if (e10 < -22 || e10 > 22) return nullptr;
+    r = e10 < 0 ? r / kPow10[-e10] : r * kPow10[e10];
+  }
+  v = neg ? -r : r;
+  return p;
+}
+
+// A value after blanks, converted as sscanf "%lf" does (strtod): Clinger's
+// fast path when it applies, else strtod on a copy of the token (the mapping
+// has no terminator).
 inline const char *parse_double(const char *s, const char *end, double &v) {
   while (s < end && blank(*s)) ++s;
+  if (const char *q = parse_decimal_fast(s, end, v)) return q;
   char buf[160];
   size_t k = 0;
   while (s + k < end && k < sizeof(buf) - 1 && s[k] != '\n' && !blank(s[k])) {

@@ -211,3 +211,58 @@ def load_csr(path: str, device: bool = True):
     ia, ja, a = np.empty(n + 1, np.int32), np.empty(nnz, np.int32), np.empty(nnz)
     _call("lap_csr_load", str(path).encode(), ia.ctypes.data, ja.ctypes.data, a.ctypes.data, 0)
     return CSR(n, ia, ja, a)
+
+
+class MMatrix(C.Structure):
+    """lap_mmatrix (include/lap_io.h): the layout of the reference's mmatrix
+    (io_utils.h:27-40)."""
+    _fields_ = [("coo_rows", C.POINTER(C.c_int)), ("coo_cols", C.POINTER(C.c_int)),
+                ("coo_vals", C.POINTER(C.c_double)), ("csr_ia", C.POINTER(C.c_int)),
+                ("csr_ja", C.POINTER(C.c_int)), ("csr_vals", C.POINTER(C.c_double)),
+                ("n", C.c_int), ("m", C.c_int), ("nnz", C.c_int), ("nnz_unpacked", C.c_int),
+                ("symmetric", C.c_int)]
+
+
+def host_load(path: str, adjacency: bool = False, weighted: int | None = None):
+    """The reference's host set-up through the library's io_utils.h
+    definitions: read_mm_file (or read_adjacency_file), coo_to_csr, then
+    split (or create_L_and_split(weighted)).  Returns (A, L, d, U) as host
+    CSR / numpy arrays."""
+    lib = L.load()
+    P = C.POINTER(MMatrix)
+    for nm in ("read_mm_file", "read_adjacency_file"):
+        getattr(lib, nm).argtypes = [C.c_char_p, P]
+        getattr(lib, nm).restype = None
+    lib.coo_to_csr.argtypes = [P]
+    lib.coo_to_csr.restype = None
+    lib.split.argtypes = [P, P, P, P]
+    lib.split.restype = None
+    lib.create_L_and_split.argtypes = [P, P, P, P, C.c_int]
+    lib.create_L_and_split.restype = None
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    A, Lm, U, D = MMatrix(), MMatrix(), MMatrix(), MMatrix()
+    (lib.read_adjacency_file if adjacency else lib.read_mm_file)(str(path).encode(), C.byref(A))
+    if A.n == 0:
+        L.check()
+        raise L.LapError(f"cannot read {path}")
+    lib.coo_to_csr(C.byref(A))
+    if weighted is None:
+        lib.split(C.byref(A), C.byref(Lm), C.byref(U), C.byref(D))
+    else:
+        lib.create_L_and_split(C.byref(A), C.byref(Lm), C.byref(U), C.byref(D), int(weighted))
+    n = A.n
+
+    def take(M, with_vals=True):
+        ia = np.ctypeslib.as_array(M.csr_ia, shape=(n + 1,)).copy()
+        k = int(ia[-1])
+        ja = np.ctypeslib.as_array(M.csr_ja, shape=(k,)).copy() if k else np.zeros(0, np.int32)
+        a = np.ctypeslib.as_array(M.csr_vals, shape=(k,)).copy() if k else np.zeros(0)
+        return CSR(n, ia, ja, a)
+
+    out = (take(A), take(Lm), np.ctypeslib.as_array(D.csr_vals, shape=(n,)).copy(), take(U))
+    for M in (A, Lm, U, D):
+        for p in (M.csr_ia, M.csr_ja, M.csr_vals):
+            if p:
+                libc.free(C.cast(p, C.c_void_p))
+    return out

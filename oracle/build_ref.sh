@@ -47,5 +47,12 @@ if [ -f "$LIB/liblapgfx.so" ]; then
     $CC $HIP_DEFS -O3 $QUIET -I"$REF" -o "$OUT/${drv#*:}" \
         $(src $SOLVER ${drv%%:*}.c) -L"$LIB" -llapgfx -Wl,-rpath,'$ORIGIN/../../lap_amd' -lm
   done
+  # the same drivers with io_utils.c left out: read_mm_file, coo_to_csr,
+  # split, create_L_and_split and read_rhs come from liblapgfx.so as well
+  NOIO="blas.c GS.c it_jacobi.c line_jacobi.c prec.c cg.c"
+  for drv in cg_driver2:lap_dropin_io_cg cg_driver:lap_dropin_io_laplacian; do
+    $CC $HIP_DEFS -O3 $QUIET -I"$REF" -o "$OUT/${drv#*:}" \
+        $(src $NOIO ${drv%%:*}.c) -L"$LIB" -llapgfx -Wl,-rpath,'$ORIGIN/../../lap_amd' -lm
+  done
 fi
 echo "build_ref: built $(ls "$OUT" | tr '\n' ' ')"

@@ -64,6 +64,41 @@ def general_with_duplicates(d, n=300, seed=3):
                   crlf=True, sep="\t")
 
 
+def value_formats(d, n=100, k=30000, seed=8):
+    """Values in every syntax strtod accepts that a file is likely to hold:
+    %.1g .. %.17g, %e, integers, signs, leading zeros, '5.', '.5', large and
+    tiny exponents, subnormals, -0, and digit strings longer than 19."""
+    rng = np.random.default_rng(seed)
+    rows, cols = rng.integers(0, n, k), rng.integers(0, n, k)
+    mags = 10.0 ** rng.uniform(-30, 30, k)
+    x = rng.choice([-1.0, 1.0], k) * mags * rng.random(k)
+    vals = []
+    fixed = ["0", "-0", "+0.0", "5.", ".5", "-.25", "007", "1e5", "1E-5", "1e+22", "1e-22",
+             "9007199254740993", "9007199254740992", "12345678901234567890", "1e23",
+             "4.9e-324", "2.2250738585072014e-308", "1.7976931348623157e308",
+             "0.000000000000000000000000123", "123456789012345678901234567890e-20",
+             "3.14159265358979323846264338327950288", "-1", "+1", "1.5e-5", "6.00000001"]
+    for i in range(k):
+        if i < len(fixed):
+            vals.append(fixed[i])
+            continue
+        f = i % 6
+        if f == 0:
+            vals.append("%.*g" % (int(rng.integers(1, 18)), x[i]))
+        elif f == 1:
+            vals.append("%.*e" % (int(rng.integers(0, 17)), x[i]))
+        elif f == 2:
+            vals.append(str(int(rng.integers(-10**6, 10**6))))
+        elif f == 3:
+            vals.append("%.*f" % (int(rng.integers(0, 12)), x[i] / mags[i] * 1000))
+        elif f == 4:
+            vals.append("%.17g" % x[i])
+        else:
+            vals.append(repr(float(x[i])))
+    return _write(os.path.join(d, "value_formats.mtx"),
+                  "%%MatrixMarket matrix coordinate real general", n, n, rows, cols, vals)
+
+
 def long_rows_symmetric(d, n=20000, seed=4):
     """Symmetric file whose mirrored entries make two rows of ~9000 and
     ~20000 entries (the multi-run merge path), plus a random sparse part."""
@@ -122,5 +157,5 @@ def rhs_file(d, n, seed=7):
     return path, b
 
 
-ALL_MATRICES = [grid_symmetric, general_with_duplicates, long_rows_symmetric]
+ALL_MATRICES = [grid_symmetric, general_with_duplicates, value_formats, long_rows_symmetric]
 ALL_ADJACENCY = [adjacency_pattern, adjacency_weighted]

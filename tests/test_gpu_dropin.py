@@ -8,7 +8,11 @@ The same for the reference's Laplacian driver (cg_driver.c): an adjacency
 file becomes the unweighted graph Laplacian (io_utils.c:80-130, :364-420),
 b = (-1)^i is projected off the constant vector (cg_driver.c:188-206) and the
 singular system is solved, with the projection's dot/scal/axpy running on
-the library too."""
+the library too.
+
+Each driver is also linked with io_utils.c left out (lap_dropin_io_*): the
+file reading, COO -> CSR, split and Laplacian construction then come from the
+library's io_utils.h definitions."""
 import os
 import re
 import subprocess
@@ -24,6 +28,9 @@ REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
 DROPIN = os.path.join(REF, "lap_dropin_cg")
 CPU = os.path.join(REF, "lap_cpu_cg")
 DROPIN_LAP = os.path.join(REF, "lap_dropin_laplacian")
+DROPIN_IO = os.path.join(REF, "lap_dropin_io_cg")
+DROPIN_IO_LAP = os.path.join(REF, "lap_dropin_io_laplacian")
+IO_VARIANTS = {"io_utils.c": (DROPIN, DROPIN_LAP), "library_io": (DROPIN_IO, DROPIN_IO_LAP)}
 CPU_LAP = os.path.join(REF, "lap_cpu_laplacian")
 
 
@@ -39,9 +46,13 @@ def run(exe, args):
 
 @pytest.mark.skipif(not (os.path.exists(DROPIN) and os.path.exists(CPU)),
                     reason="drop-in binary not built (needs the reference sources)")
+@pytest.mark.parametrize("io", sorted(IO_VARIANTS))
 @pytest.mark.parametrize("kind", ["line_jacobi", "GS_std", "ichol", "it_jacobi"])
-def test_reference_driver_on_gfx950_backend(tmp_path, kind):
-    print(f"[case] dropin kind={kind}", flush=True)
+def test_reference_driver_on_gfx950_backend(tmp_path, kind, io):
+    print(f"[case] dropin kind={kind} io={io}", flush=True)
+    exe = IO_VARIANTS[io][0]
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
     A = grid_laplacian([128, 128], 1e-8)
     b = rhs_uniform(A.n, 12345)
     mm, rhs = str(tmp_path / "A.mtx"), str(tmp_path / "b.mtx")
@@ -49,7 +60,7 @@ def test_reference_driver_on_gfx950_backend(tmp_path, kind):
     write_rhs(rhs, b)
     k = "2" if kind == "it_jacobi" else "1"
     args = [mm, kind, "1e-8", "25000", "1", k, rhs]
-    gi, gres, greason, gtext = run(DROPIN, args)
+    gi, gres, greason, gtext = run(exe, args)
     ci, cres, creason, _ = run(CPU, args)
     print(f"[diag] dropin {kind}: gpu it {gi} res {gres:.6e} ({greason}); "
           f"cpu it {ci} res {cres:.6e} ({creason})", flush=True)
@@ -67,16 +78,20 @@ GRAPHS = {
 
 @pytest.mark.skipif(not (os.path.exists(DROPIN_LAP) and os.path.exists(CPU_LAP)),
                     reason="Laplacian drop-in binary not built (needs the reference sources)")
+@pytest.mark.parametrize("io", sorted(IO_VARIANTS))
 @pytest.mark.parametrize("kind", ["line_jacobi", "GS_std", "it_jacobi", "GS_it2"])
 @pytest.mark.parametrize("graph", sorted(GRAPHS))
-def test_reference_laplacian_driver_on_gfx950_backend(tmp_path, graph, kind):
-    print(f"[case] dropin laplacian graph={graph} kind={kind}", flush=True)
+def test_reference_laplacian_driver_on_gfx950_backend(tmp_path, graph, kind, io):
+    print(f"[case] dropin laplacian graph={graph} kind={kind} io={io}", flush=True)
+    exe = IO_VARIANTS[io][1]
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
     n, src, dst = GRAPHS[graph]()
     mm = str(tmp_path / "adj.mtx")
     write_adjacency(mm, n, src, dst)
     k = "2" if kind in ("it_jacobi", "GS_it2") else "1"
     args = [mm, kind, "1e-10", "25000", "1", k]
-    gi, gres, greason, gtext = run(DROPIN_LAP, args)
+    gi, gres, greason, gtext = run(exe, args)
     ci, cres, creason, _ = run(CPU_LAP, args)
     print(f"[diag] dropin laplacian {graph}/{kind}: gpu it {gi} res {gres:.6e} ({greason}); "
           f"cpu it {ci} res {cres:.6e} ({creason})", flush=True)
