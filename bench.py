@@ -206,6 +206,39 @@ def bench_batched(S, torch, np, count: int = 256, tol: float = 1e-10) -> dict:
     }
 
 
+def bench_sweeps(S, torch, A, Ah, b, tol: float) -> dict:
+    """The 256^3 system with the SpMV-only preconditioners at M/K values of
+    the reference's experiment table (script_hip/run_power_cg_hip.sh):
+    it_jacobi K=2, GS_it2 M=1 K=1 and M=1 K=3, each solved with one launch
+    per sweep (fuse_sweeps=1, the default) and with the reference's launch
+    sequence (fuse_sweeps=0); both give the same iterates bit for bit."""
+    out = {}
+    for kind, m, k in (("it_jacobi", 1, 2), ("GS_it2", 1, 1), ("GS_it2", 1, 3)):
+        P = S.make_preconditioner(A, Ah, kind, m=m, k=k)
+        res = {}
+        try:
+            for fuse in (0, 1):
+                S.set_option("fuse_sweeps", fuse)
+                S.pcg(A, b, P, tol=tol, maxit=25000)  # warm (graph, plans)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = S.pcg(A, b, P, tol=tol, maxit=25000)
+                res[fuse] = (time.perf_counter() - t0, r)
+        finally:
+            S.set_option("fuse_sweeps", 1)
+        (w0, r0), (w1, r1) = res[0], res[1]
+        out[f"{kind}_M{m}_K{k}"] = {
+            "value": w1, "unit": "s", "iterations": r1.iterations, "flag": r1.flag,
+            "ms_per_iteration": r1.gpu_ms / max(r1.iterations, 1),
+            "unfused_s": w0, "unfused_ms_per_iteration": r0.gpu_ms / max(r0.iterations, 1),
+            "speedup_vs_unfused": w0 / w1,
+            "bitwise_equal_to_unfused": bool(torch.equal(r0.x, r1.x)
+                                             and r0.iterations == r1.iterations),
+        }
+        del P
+    return out
+
+
 def bench_adjoint(torch, np, n: int = 1_000_000, tol: float = 1e-10) -> dict:
     """Forward + adjoint solve of L_w x = b on a random geometric graph (n
     points U[0,1]^2, r = sqrt(10/(pi n)), w = exp(-d^2/sigma^2), sigma = r/2)
@@ -530,6 +563,7 @@ def main() -> int:
     if not args.no_extras:
         for name, fn in (("preconditioners",
                           lambda: bench_preconditioners(S, torch, A, Ah, b, args.tol)),
+                         ("sweeps", lambda: bench_sweeps(S, torch, A, Ah, b, args.tol)),
                          ("block", lambda: bench_block(S, torch, np)),
                          ("batched", lambda: bench_batched(S, torch, np)),
                          ("adjoint", lambda: bench_adjoint(torch, np)),

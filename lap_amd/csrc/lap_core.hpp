@@ -182,6 +182,7 @@ struct Options {
                            // 1 one sync-free launch
   int sf_wg_per_cu = 8;    // sync-free solve: workgroups per CU
   int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
+  int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
                            // 1 three kernels, x in K_C; 2 two kernels (p in the
                            // SpMV gather)
@@ -216,6 +217,27 @@ void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *
                  const int *done = nullptr);
 // Persistent SpMV grid: min(row blocks, 8 workgroups per CU).
 unsigned spmv_grid(const SpmvPlan &p);
+
+// --------------------------------------------------- SpMV-only sweeps
+// Row blocks over a pair of CSR matrices (the strict L and U of a split),
+// at most kSpmvTile nonzeros of both per block (a longer row gets a block of
+// its own).
+struct PairPlan {
+  int n = 0;
+  int nblocks = 0;
+  DevBufPtr rowblk;  // int[nblocks + 1]: first row of each block
+};
+const PairPlan *pair_plan(int n, int nl, const int *lia, int nu, const int *uia);
+// y_i = base_i - sum_j a_ij x_j (CSR order), stored per `mode`:
+// 0 out = y * d_r; 1 out = y; 2 out = y, out2 = y * d_r;
+// 3 out2 = (first ? 0 : out2) + y * d_r.
+void sweep_one_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *a,
+                      const double *x, const double *base, const double *d_r, double *out,
+                      double *out2, int mode, bool first, const int *done);
+// out_i = (base_i - (sum_U a_ij x_j + sum_L a_ij x_j)) * d_r_i
+void sweep_pair_launch(const PairPlan &p, const int *lia, const int *lja, const double *la,
+                       const int *uia, const int *uja, const double *ua, const double *x,
+                       const double *base, const double *d_r, double *out, const int *done);
 
 // --------------------------------------------------- triangular solve plan
 struct TriPlan {

@@ -605,6 +605,7 @@ struct PrecOps {
   const double *a = nullptr;
   lap_pdata *pd = nullptr;
   const SpmvPlan *A = nullptr, *Ls = nullptr, *Us = nullptr;  // SpMV plans
+  const PairPlan *LU = nullptr;                               // L and U together
   const TriPlan *Lt = nullptr, *Ut = nullptr;                 // triangular plans
   const double *ldiag = nullptr, *udiag = nullptr;
   const IcPlan *ic = nullptr;
@@ -666,6 +667,7 @@ static bool prec_setup(PrecOps &o, int n, int nnzA, const int *ia, const int *ja
     o.Ls = spmv_plan(n, pd->lnnz, pd->lia, pd->lja);
     o.Us = spmv_plan(n, pd->unnz, pd->uia, pd->uja);
     if (!o.Ls || !o.Us) return false;
+    if (o.kind != P_GS_IT) o.LU = pair_plan(n, pd->lnnz, pd->lia, pd->unnz, pd->uia);
   }
   if (o.kind == P_GS_STD) {
     if (!check_device("GS_std", {{pd->lia, "prec_data->lia"}, {pd->lja, "prec_data->lja"},
@@ -702,10 +704,94 @@ static void spmv_u(const PrecOps &o, const double *x, double *y, double al, doub
   spmv_launch(*o.Us, o.pd->uia, o.pd->uja, o.pd->ua, x, y, al, be, done);
 }
 
+// The SpMV-only preconditioners with one launch per sweep (sweep.hip).
+// Every value is bitwise the unfused sequence in prec_launch for finite
+// operands, with two shortcuts that use x = 0: it_jacobi's first sweep is
+// d_r .* r, and GS_it's first residual r - A*0 is r itself (the unfused sum
+// can turn a -0.0 entry of r into +0.0; no value changes).  Returns false
+// (the caller runs the unfused sequence) for M or K below 1 and when w
+// shares memory with r or an aux vector.
+static bool prec_fused(const PrecOps &o, const double *r, double *w, const int *done) {
+  if (!options().fuse_sweeps || o.k < 1 || o.m < 1) return false;
+  if (o.kind != P_IT_JACOBI && o.kind != P_GS_IT && o.kind != P_GS_IT2) return false;
+  if (w == r || w == o.aux1 || w == o.aux2 || w == o.aux3) return false;
+  if (o.kind != P_GS_IT && !o.LU) return false;
+  const int n = o.n, K = o.k;
+  lap_pdata *pd = o.pd;
+  const double *dr = pd->d_r;
+  auto pair = [&](const double *x, const double *base, double *out) {
+    sweep_pair_launch(*o.LU, pd->lia, pd->lja, pd->la, pd->uia, pd->uja, pd->ua, x, base, dr,
+                      out, done);
+  };
+  auto one_l = [&](const double *x, const double *base, double *out, double *out2, int mode,
+                   bool first) {
+    sweep_one_launch(*o.Ls, pd->lia, pd->lja, pd->la, x, base, dr, out, out2, mode, first, done);
+  };
+  auto one_u = [&](const double *x, const double *base, double *out, double *out2, int mode,
+                   bool first) {
+    sweep_one_launch(*o.Us, pd->uia, pd->uja, pd->ua, x, base, dr, out, out2, mode, first, done);
+  };
+  if (o.kind == P_IT_JACOBI) {  // it_jacobi.c:4-32
+    // sweep s (1..K) writes w when K - s is even, else aux1, so sweep K
+    // ends in w
+    auto dst = [&](int s) { return ((K - s) % 2 == 0) ? w : o.aux1; };
+    launch_vec_vec(n, r, dr, dst(1), done);
+    for (int s = 2; s <= K; ++s) pair(dst(s - 1), r, dst(s));
+    return true;
+  }
+  if (o.kind == P_GS_IT) {  // GS.c:45-92
+    for (int j = 0; j < o.m; ++j) {
+      const double *base = r;  // r - A w (w = 0 at j = 0)
+      if (j == 0)
+        launch_vec_vec(n, r, dr, o.aux1, done);
+      else {
+        sweep_one_launch(*o.A, o.ia, o.ja, o.a, w, r, dr, o.aux2, o.aux1, 2, false, done);
+        base = o.aux2;
+      }
+      double *y = o.aux1, *alt = o.aux3;
+      for (int s = 0; s < 2 * K; ++s) {
+        const bool lower = s < K;
+        if (s == 2 * K - 1) {  // last U sweep: w += y .* d_r
+          one_u(y, base, nullptr, w, 3, j == 0);
+        } else {
+          if (lower)
+            one_l(y, base, alt, nullptr, 0, false);
+          else
+            one_u(y, base, alt, nullptr, 0, false);
+          std::swap(y, alt);
+        }
+      }
+    }
+    return true;
+  }
+  // GS_it2, GS.c:95-159: y ping-pongs between aux1 and aux2, aux3 holds the
+  // residuals r - L y and r - U y; the last sweep writes w
+  double *buf[2] = {o.aux1, o.aux2};
+  int cur = 0;
+  launch_vec_vec(n, r, dr, buf[0], done);
+  for (int j = 0; j < o.m; ++j) {
+    pair(buf[cur], r, buf[cur ^ 1]);
+    cur ^= 1;
+    one_l(buf[cur], r, o.aux3, nullptr, 1, false);
+    for (int s = 0; s < K; ++s) {
+      one_u(buf[cur], o.aux3, buf[cur ^ 1], nullptr, 0, false);
+      cur ^= 1;
+    }
+    one_u(buf[cur], r, o.aux3, nullptr, 1, false);
+    for (int s = 0; s < K; ++s) {
+      const bool last = j == o.m - 1 && s == K - 1;
+      one_l(buf[cur], o.aux3, last ? w : buf[cur ^ 1], nullptr, 0, false);
+      cur ^= 1;
+    }
+  }
+  return true;
+}
+
 // w = M^{-1} r, as launches on the library stream.
 static void prec_launch(const PrecOps &o, const double *r, double *w, const int *done) {
   const int n = o.n;
   lap_pdata *pd = o.pd;
+  if (prec_fused(o, r, w, done)) return;
   switch (o.kind) {
     case P_NONE:
       launch_copy(n, r, w, done);

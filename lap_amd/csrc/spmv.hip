@@ -152,6 +152,8 @@ void forget_ic_plans(const void *p);
 void forget_mix_plans(const void *p);
 void forget_batch_plans(const void *p);
 void forget_block_plans(const void *p);
+void forget_pair_plans(const void *p);
+void release_pair_plans();
 
 void forget_plans(const void *p) {
   for (auto it = g_plans.begin(); it != g_plans.end();) {
@@ -165,8 +167,10 @@ void forget_plans(const void *p) {
   forget_mix_plans(p);
   forget_batch_plans(p);
   forget_block_plans(p);
+  forget_pair_plans(p);
 }
 void release_plans() {
+  release_pair_plans();
   release_block_plans();
   release_batch_plans();
   release_mix_plans();

@@ -1,0 +1,60 @@
+"""Laplacian mode on the device (lap_amd.laplacian): the reference driver
+cg_driver.c's pipeline -- adjacency file, unweighted or normalised graph
+Laplacian, b = (-1)^i projected off the null space, PCG on the singular
+system -- built and solved on the device.
+
+Bars: the exit reason equals the reference's and the iteration count is
+within +-1 of it (the dot products' summation order differs); the solution
+agrees with the reference's to 1e-8 relative after removing its component
+along the null vector.  The reference is its own CPU path: for the
+unweighted Laplacian the lap_cpu_laplacian binary (cg_driver.c as built by
+MakefileCPU) and the same pipeline composed from the reference's compiled
+functions (read_adjacency_file, coo_to_csr, create_L_and_split, dot, scal,
+axpy, vector_sqrt, vec_vec, cg); for the normalised Laplacian, which the
+driver hard-codes off (cg_driver.c:46), only the composed pipeline.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from lap_amd import laplacian as LP
+from lap_amd.problems import write_adjacency
+from lapref import GRAPHS, null_free, reference_pipeline
+
+pytestmark = pytest.mark.gpu
+
+REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref")
+CPU_LAP = os.path.join(REF, "lap_cpu_laplacian")
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("kind,m,k", [("line_jacobi", 1, 1), ("GS_std", 1, 1),
+                                      ("it_jacobi", 1, 2), ("GS_it2", 1, 2)])
+@pytest.mark.parametrize("graph", sorted(GRAPHS))
+def test_laplacian_mode_matches_reference(ref, tmp_path, graph, kind, m, k, weighted):
+    print(f"[case] laplacian {graph} {kind} M={m} K={k} weighted={weighted}", flush=True)
+    n, src, dst = GRAPHS[graph]()
+    path = str(tmp_path / "adj.mtx")
+    write_adjacency(path, n, src, dst)
+    tol = 1e-10
+    P = LP.LaplacianProblem.from_adjacency(path, weighted=weighted, prec=kind, m=m, k=k)
+    res = P.solve(tol=tol)
+    xr, itr, flr, _ = reference_pipeline(ref, path, weighted, kind, m, k, tol)
+    d = P.d.cpu().numpy()
+    x = res.x.cpu().numpy()
+    xa, xb = null_free(x, d, weighted), null_free(xr, d, weighted)
+    err = np.linalg.norm(xa - xb) / np.linalg.norm(xb)
+    print(f"[diag] laplacian {graph}/{kind} weighted={weighted}: gpu it {res.iterations} "
+          f"flag {res.flag}; reference it {itr} flag {flr}; rel err {err:.2e}", flush=True)
+    assert res.flag == flr == 0
+    assert abs(res.iterations - itr) <= 1
+    assert err <= 1e-8
+    if not weighted and os.path.exists(CPU_LAP):
+        out = subprocess.run([CPU_LAP, path, kind, str(tol), "25000", str(m), str(k)],
+                             capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0
+        it_bin = int(re.search(r"Iters\s*:\s*(\d+)", out.stdout).group(1))
+        assert it_bin == itr  # the composed pipeline is the driver
