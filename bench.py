@@ -239,6 +239,33 @@ def bench_sweeps(S, torch, A, Ah, b, tol: float) -> dict:
     return out
 
 
+def bench_energy(S, torch, A, b, P, tol: float, solves: int = 3) -> dict:
+    """Energy to solution of the FP64 and mixed-precision Jacobi solves on the
+    256^3 system: the GPU's energy accumulator read around `solves`
+    back-to-back solves, the socket power sampled every 20 ms beside it (the
+    reference's time x mean-power figure), and the idle power over 1 s."""
+    from lap_amd.energy import EnergyMeter
+    out = {}
+    with EnergyMeter() as idle:
+        time.sleep(1.0)
+    if not idle.available:
+        return idle.report()
+    out["idle"] = idle.report()
+    for name, fn in (("fp64", lambda: S.pcg(A, b, P, tol=tol, maxit=25000)),
+                     ("mixed", lambda: S.pcg_mixed(A, b, P, tol=tol, maxit=25000))):
+        fn()
+        torch.cuda.synchronize()
+        with EnergyMeter() as em:
+            for _ in range(solves):
+                fn()
+            torch.cuda.synchronize()
+        out[name] = em.report(solves)
+    if "fp64" in out and "mixed" in out and "energy_j_per_solve" in out["mixed"]:
+        out["mixed_energy_saving"] = (out["fp64"]["energy_j_per_solve"]
+                                      / out["mixed"]["energy_j_per_solve"])
+    return out
+
+
 def bench_adjoint(torch, np, n: int = 1_000_000, tol: float = 1e-10) -> dict:
     """Forward + adjoint solve of L_w x = b on a random geometric graph (n
     points U[0,1]^2, r = sqrt(10/(pi n)), w = exp(-d^2/sigma^2), sigma = r/2)
@@ -563,6 +590,7 @@ def main() -> int:
     if not args.no_extras:
         for name, fn in (("preconditioners",
                           lambda: bench_preconditioners(S, torch, A, Ah, b, args.tol)),
+                         ("energy", lambda: bench_energy(S, torch, A, b, P, args.tol)),
                          ("sweeps", lambda: bench_sweeps(S, torch, A, Ah, b, args.tol)),
                          ("block", lambda: bench_block(S, torch, np)),
                          ("batched", lambda: bench_batched(S, torch, np)),

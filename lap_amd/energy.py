@@ -1,0 +1,127 @@
+"""Energy to solution on the GPU (the reference's power-sampling methodology,
+script_hip/continous.sh:5 and script/analyzeEnergy_cuda.py:183-201).
+
+    with EnergyMeter() as em:
+        ... timed GPU work ...
+    em.joules, em.mean_watts, em.seconds
+
+The reference samples `rocm-smi --csv -P` once a second while the solver
+runs and reports energy = solve time x mean power.  Here the GPU's own
+energy accumulator is read before and after the region through the AMD SMI
+library (amdsmi_get_energy_count: microjoule counter x resolution), and the
+socket power is sampled every `period` seconds on a thread beside it, so
+both the accumulator's energy and the reference's time x mean-power figure
+are reported.  When the AMD SMI library cannot be loaded or does not expose
+the counters, the meter reports itself unavailable (no fallback figure).
+"""
+from __future__ import annotations
+
+import os
+import sys
+import threading
+import time
+
+_SMI_DIR = "/opt/rocm/share/amd_smi"
+
+
+def _our_bus():
+    """PCI bus number of the GPU torch uses as device 0, or None."""
+    try:
+        import torch
+        return int(torch.cuda.get_device_properties(0).pci_bus_id)
+    except Exception:
+        return None
+
+
+def _load_smi():
+    """(amdsmi module, handle of our GPU, bdf string, error)."""
+    try:
+        if os.path.isdir(_SMI_DIR) and _SMI_DIR not in sys.path:
+            sys.path.append(_SMI_DIR)
+        import amdsmi  # noqa: PLC0415
+        amdsmi.amdsmi_init()
+        handles = amdsmi.amdsmi_get_processor_handles()
+        if not handles:
+            return None, None, None, "no GPU handle"
+        bus = _our_bus()
+        for h in handles:  # the handle whose PCI bus is torch's device 0
+            bdf = amdsmi.amdsmi_get_gpu_device_bdf(h)
+            if len(handles) == 1 or (bus is not None and int(bdf.split(":")[1], 16) == bus):
+                return amdsmi, h, bdf, None
+        return None, None, None, f"device 0 (bus {bus}) not among {len(handles)} GPUs"
+    except Exception as exc:  # library absent, no permission, no device
+        return None, None, None, f"{type(exc).__name__}: {exc}"
+
+
+class EnergyMeter:
+    def __init__(self, period: float = 0.02):
+        self.period = period
+        self.smi, self.h, self.bdf, self.error = _load_smi()
+        self.samples: list[tuple[float, float]] = []
+        self.joules = None
+        self.seconds = None
+        self.mean_watts = None
+        self._stop = threading.Event()
+        self._thread = None
+
+    @property
+    def available(self) -> bool:
+        return self.smi is not None
+
+    def _energy_j(self):
+        c = self.smi.amdsmi_get_energy_count(self.h)
+        return c["energy_accumulator"] * c["counter_resolution"] * 1e-6
+
+    def _power_w(self):
+        p = self.smi.amdsmi_get_power_info(self.h)
+        for key in ("current_socket_power", "socket_power", "average_socket_power"):
+            v = p.get(key)
+            if isinstance(v, (int, float)) and v not in (0, 0xFFFF):
+                return float(v)
+        return None
+
+    def _sample(self):
+        while not self._stop.is_set():
+            try:
+                w = self._power_w()
+                if w is not None:
+                    self.samples.append((time.perf_counter(), w))
+            except Exception:
+                pass
+            self._stop.wait(self.period)
+
+    def __enter__(self):
+        if not self.available:
+            return self
+        self._e0 = self._energy_j()
+        self._t0 = time.perf_counter()
+        self._thread = threading.Thread(target=self._sample, daemon=True)
+        self._thread.start()
+        return self
+
+    def __exit__(self, *exc):
+        if not self.available:
+            return False
+        t1 = time.perf_counter()
+        e1 = self._energy_j()
+        self._stop.set()
+        self._thread.join()
+        self.seconds = t1 - self._t0
+        self.joules = e1 - self._e0
+        if self.samples:
+            self.mean_watts = sum(w for _, w in self.samples) / len(self.samples)
+        return False
+
+    def report(self, solves: int = 1) -> dict:
+        if not self.available:
+            return {"error": f"AMD SMI energy counters unavailable ({self.error})"}
+        out = {
+            "energy_j_per_solve": self.joules / solves,
+            "mean_power_w_counter": self.joules / self.seconds,
+            "seconds": self.seconds, "solves": solves,
+            "power_samples": len(self.samples), "gpu_bdf": self.bdf,
+        }
+        if self.mean_watts is not None:
+            out["mean_power_w_sampled"] = self.mean_watts
+            out["energy_j_per_solve_time_x_power"] = self.mean_watts * self.seconds / solves
+        return out

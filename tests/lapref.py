@@ -52,7 +52,7 @@ def reference_pipeline(ref, path, weighted, kind, m, k, tol):
     return x, it, flag, hist
 
 
-def null_free(x, d, weighted):
-    v = np.sqrt(d) if weighted else np.ones_like(x)
+def null_free(x, v):
+    """x without its component along the null vector v."""
     v = v / np.linalg.norm(v)
     return x - (v @ x) * v

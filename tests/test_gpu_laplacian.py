@@ -6,7 +6,14 @@ system -- built and solved on the device.
 Bars: the exit reason equals the reference's and the iteration count is
 within +-1 of it (the dot products' summation order differs); the solution
 agrees with the reference's to 1e-8 relative after removing its component
-along the null vector.  The reference is its own CPU path: for the
+along the null vector (ones, or D^1/2 ones for the normalised Laplacian).
+The driver's normalised mode projects b off sqrt(diag(L)) = ones instead of
+D^1/2 ones (cg_driver.c:209-229, d = 1 there), so unless b happens to be
+orthogonal to D^1/2 ones (the grid) the system is inconsistent: the solution
+then grows without bound along the null vector in the reference as here, and
+only the exit reason, the blow-up in both and the first residual-history
+entries are compared (the exit iteration itself depends on which dip of the
+oscillating residual first crosses tol: 96 against 123 for it_jacobi).  The reference is its own CPU path: for the
 unweighted Laplacian the lap_cpu_laplacian binary (cg_driver.c as built by
 MakefileCPU) and the same pipeline composed from the reference's compiled
 functions (read_adjacency_file, coo_to_csr, create_L_and_split, dot, scal,
@@ -42,16 +49,33 @@ def test_laplacian_mode_matches_reference(ref, tmp_path, graph, kind, m, k, weig
     tol = 1e-10
     P = LP.LaplacianProblem.from_adjacency(path, weighted=weighted, prec=kind, m=m, k=k)
     res = P.solve(tol=tol)
-    xr, itr, flr, _ = reference_pipeline(ref, path, weighted, kind, m, k, tol)
-    d = P.d.cpu().numpy()
+    xr, itr, flr, hr = reference_pipeline(ref, path, weighted, kind, m, k, tol)
+    # null vector: ones, or D^1/2 ones for the normalised Laplacian (D the
+    # int degrees).  The driver projects b off sqrt(diag) = ones in both
+    # modes (cg_driver.c:209-229 with d = 1), so in the normalised case b
+    # keeps a component along D^1/2 ones and x's component there is not
+    # determined; it is removed before comparing.
+    v = np.sqrt(P.deg.cpu().numpy().astype(np.float64)) if weighted else np.ones(P.A.n)
     x = res.x.cpu().numpy()
-    xa, xb = null_free(x, d, weighted), null_free(xr, d, weighted)
+    b = P.b.cpu().numpy()
+    consistent = abs(b @ v) <= 1e-12 * np.linalg.norm(b) * np.linalg.norm(v)
+    xa, xb = null_free(x, v), null_free(xr, v)
     err = np.linalg.norm(xa - xb) / np.linalg.norm(xb)
     print(f"[diag] laplacian {graph}/{kind} weighted={weighted}: gpu it {res.iterations} "
           f"flag {res.flag}; reference it {itr} flag {flr}; rel err {err:.2e}", flush=True)
     assert res.flag == flr == 0
-    assert abs(res.iterations - itr) <= 1
-    assert err <= 1e-8
+    if consistent:
+        assert abs(res.iterations - itr) <= 1
+        assert err <= 1e-8
+    else:
+        # b keeps a component along the null vector: x grows along it without
+        # bound (~1e13 here, in the reference as on the device), the
+        # recurrence residual swings over nine orders of magnitude before it
+        # drops below tol, and solutions are not comparable
+        assert weighted and abs(x @ v) / np.linalg.norm(v) > 1e6 * np.linalg.norm(b)
+        assert abs(xr @ v) / np.linalg.norm(v) > 1e6 * np.linalg.norm(b)
+        h = res.history[:4]
+        assert np.all(np.abs(h - hr[:4]) <= 1e-8 * hr[:4])
     if not weighted and os.path.exists(CPU_LAP):
         out = subprocess.run([CPU_LAP, path, kind, str(tol), "25000", str(m), str(k)],
                              capture_output=True, text=True, timeout=300)
