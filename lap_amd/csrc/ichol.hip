@@ -243,7 +243,7 @@ void hip_ichol(const int *ia, const int *ja, double *a, const int nnzA, lap_pdat
                                                    : scratch_vec("ichol_aux", p->n);
   ic_apply_launch(*p, x, y, aux, nullptr);
   sync();
-  if (options().sptrsv_mode == 1 && tri_error()) {
+  if (options().sptrsv_mode != 0 && tri_error()) {
     set_error(LAP_ERR_NUMERIC, "hip_ichol: a triangular-solve dependency wait ran out");
     tri_clear_error();
   }

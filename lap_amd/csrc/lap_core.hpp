@@ -149,12 +149,36 @@ __device__ __forceinline__ double sum_partials(const double *partials, int cnt, 
 //                   [w*C, (w+1)*C), C = ceil(nblk/G).  On a grid-ordered
 //                   matrix the rows that share x entries (neighbouring lines
 //                   and planes) then sit in chunks of workgroups on one XCD.
+//  modes 3/4/5 XCD groups: blocks are dealt to the XCDs in groups of g
+//                   consecutive blocks (g = 4, 8, 2), group q to XCD q mod 8,
+//                   and each XCD's workgroups walk their groups cyclically.
+//                   Neighbouring blocks mostly share an XCD, and blocks
+//                   8*g*m apart always do, while every XCD still streams
+//                   addresses spread over the whole matrix.
+// Loops run s over [begin, end) with stride step and process block(s),
+// skipping block(s) >= nblk (only the last group can overrun).
 struct BlockWalk {
   int begin, end, step;
-  int nblk;  // row blocks of the plan (the nonzero offsets follow rowblk)
+  int nblk;     // row blocks of the plan (the nonzero offsets follow rowblk)
+  int g = 0;    // group size of modes 3-5 (0: s is the block)
+  int gk = 0;   // first block of this XCD's first group
+  __device__ __forceinline__ int block(int s) const {
+    return g ? (s / g) * (8 * g) + gk + (s % g) : s;
+  }
 };
 __device__ __forceinline__ BlockWalk block_walk(int nblk, int mode) {
   const int g = gridDim.x, id = blockIdx.x;
+  if (mode >= 3 && mode <= 5 && g >= 8) {
+    const int gs = mode == 3 ? 4 : mode == 4 ? 8 : 2;
+    const int k = id % 8, lw = id / 8;
+    const int nw = g / 8 + (k < g % 8 ? 1 : 0);
+    const int base = gs * k;
+    const int groups = base < nblk ? (nblk - base + 8 * gs - 1) / (8 * gs) : 0;
+    BlockWalk w{lw, groups * gs, nw, nblk};
+    w.g = gs;
+    w.gk = base;
+    return w;
+  }
   if (mode == 2) {
     const int chunk = (nblk + g - 1) / g;
     const long long b0 = (long long)id * chunk;
@@ -172,15 +196,17 @@ __device__ __forceinline__ BlockWalk block_walk(int nblk, int mode) {
 // -------------------------------------------------------------- options
 struct Options {
   int spmv_rows = 256;   // rows per SpMV block (<= kBlock * 2)
-  int spmv_walk = 0;     // row-block walk of the persistent SpMV grid (0,1,2)
+  int spmv_walk = 0;     // row-block walk of the persistent SpMV grid (0..5)
   int vec_grid = 1024;   // workgroups of the streaming vector kernels
   int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;
   int spmm_wg_per_cu = 4;  // block CG SpMM grid: workgroups per CU
   int sptrsv_mode = 0;     // triangular solves: 0 one launch per level,
-                           // 1 one sync-free launch
+                           // 1 one sync-free launch, 2 one launch with a
+                           // grid barrier per level
   int sf_wg_per_cu = 8;    // sync-free solve: workgroups per CU
+  int tb_block = 256;      // level-barrier solve: workgroup size (256 or 1024)
   int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
   int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
@@ -245,6 +271,7 @@ struct TriPlan {
   bool lower = true;
   int nlevels = 0;
   std::vector<int> level_ptr;  // host, nlevels + 1
+  DevBufPtr level_ptr_dev;     // the same on the device (level-barrier mode)
   DevBufPtr order;             // int[n]: rows sorted by level
   int max_level_rows = 0;
   bool has_diag = true;        // every row holds its pivot

@@ -172,9 +172,12 @@ __global__ __launch_bounds__(BS) void k_mx_spmv_pq(int nblk, const int *__restri
   __shared__ double red[BS / kWave];
   Pq32Epi epi{q, p, diag, 0.0};
   const BlockWalk w = block_walk(nblk, walk);
-  for (int blk = w.begin; blk < w.end; blk += w.step)
+  for (int s = w.begin; s < w.end; s += w.step) {
+    const int blk = w.block(s);
+    if (blk >= nblk) continue;
     spmv_block_rows<BS, TILE, Pq32Epi, float, float>(blk, rowblk, ia, ja, a, p, 1.0, prod, red,
                                                       epi);
+  }
   const double t = block_sum<BS>(epi.acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }

@@ -424,8 +424,11 @@ __global__ __launch_bounds__(BS) void k_jac_fused(int nblk, const int *__restric
   const double beta = rz / ctl->rho;
   FusedEpi epi{PnewX{p_old, r, d_r, beta}, p_new, q, x, alpha_x, 0.0};
   const BlockWalk w = block_walk(nblk, xcd);
-  for (int b = w.begin; b < w.end; b += w.step)
+  for (int s = w.begin; s < w.end; s += w.step) {
+    const int b = w.block(s);
+    if (b >= nblk) continue;
     spmv_block_rows_fx<BS, TILE>(b, rowblk, ia, ja, a, epi.pn, prod, red, epi);
+  }
   const double t = block_sum<BS>(epi.acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -919,7 +922,7 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
 
   double *R0 = ws.partials + kR0, *R1 = ws.partials + kR1, *R2 = ws.partials + kR2,
          *R3 = ws.partials + kR3;
-  const bool sync_free = options().sptrsv_mode == 1 &&
+  const bool sync_free = options().sptrsv_mode != 0 &&
                          (o.kind == P_ICHOL || o.kind == P_GS_STD);
   if (sync_free) tri_clear_error();
   PcgCtl h0{};
@@ -983,8 +986,11 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   // the next launch)
   const int *poll = (fused || xin_c) ? &ctl->pend_done : done;
   int per_iter = fused ? 2 : jac ? 3 : 6;
-  if (o.kind == P_ICHOL) per_iter += o.ic->lplan->nlevels + o.ic->uplan->nlevels;
-  if (o.kind == P_GS_STD) per_iter += (o.Lt->nlevels + o.Ut->nlevels + 4) * o.k;
+  const bool one_launch_solves = options().sptrsv_mode != 0;  // 2 launches per solve
+  if (o.kind == P_ICHOL)
+    per_iter += one_launch_solves ? 4 : o.ic->lplan->nlevels + o.ic->uplan->nlevels;
+  if (o.kind == P_GS_STD)
+    per_iter += ((one_launch_solves ? 4 : o.Lt->nlevels + o.Ut->nlevels) + 4) * o.k;
   if (o.kind == P_IT_JACOBI || o.kind == P_GS_IT || o.kind == P_GS_IT2)
     per_iter += 8 * o.m * (o.k + 2);
   int G = std::max(1, std::min(options().graph_iters, 4096 / per_iter));
@@ -1210,7 +1216,7 @@ void prec_function(int *ia, int *ja, double *a, int nnzA, lap_pdata *prec_data, 
   if (!prec_setup(o, prec_data->n, nnzA, ia, ja, a, prec_data)) return;
   prec_launch(o, x, y, nullptr);
   sync();
-  if (options().sptrsv_mode == 1 && tri_error()) {
+  if (options().sptrsv_mode != 0 && tri_error()) {
     set_error(LAP_ERR_NUMERIC, "prec_function: a triangular-solve dependency wait ran out");
     tri_clear_error();
   }

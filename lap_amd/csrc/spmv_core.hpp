@@ -287,14 +287,20 @@ __device__ __forceinline__ void spmv_walk(BlockWalk w, const int *__restrict__ r
                                           const double *__restrict__ a,
                                           const double *__restrict__ x, double alpha,
                                           double *prod, double *red, Epi &epi) {
-  if constexpr (V == 0 || V == 5) {
-    for (int b = w.begin; b < w.end; b += w.step)
+  if (V == 0 || V == 5 || (w.g && V != 6)) {  // the staged walks need a plain stride
+    for (int s = w.begin; s < w.end; s += w.step) {
+      const int b = w.block(s);
+      if (b >= w.nblk) continue;
       spmv_block_rows<BS, TILE, Epi, double, double, V == 5>(b, rowblk, ia, ja, a, x, alpha, prod,
                                                              red, epi);
+    }
   } else if constexpr (V == 6) {
     const int *ebl = rowblk + w.nblk + 1;
-    for (int b = w.begin; b < w.end; b += w.step)
+    for (int s = w.begin; s < w.end; s += w.step) {
+      const int b = w.block(s);
+      if (b >= w.nblk) continue;
       spmv_block_rows_pf<BS, TILE>(b, rowblk, ebl, ia, ja, a, x, alpha, prod, red, epi);
+    }
   } else if constexpr (V <= 2) {
     spmv_walk_staged<BS, TILE, V == 2>(w, rowblk, ia, ja, a, x, alpha, prod, red, epi);
   } else {

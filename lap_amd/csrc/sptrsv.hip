@@ -24,6 +24,18 @@
 // and the host reports the failure, so the kernel always ends.  The
 // arithmetic is the level-scheduled kernel's, in the same order: results
 // are bitwise identical.
+//
+// Level-barrier mode (option sptrsv_mode 2): one persistent launch per
+// solve, a grid-wide barrier between levels.  Every workgroup takes its
+// share of the current level's rows (one lane per row), stores y with
+// agent-scope stores (written through to the coherence point, so no cache
+// write-back is needed), waits until its stores are complete, and arrives at
+// a counter; wave 0 then polls the counter (wave-uniform, bounded) until all
+// workgroups have arrived, and the next level's dependencies are read with
+// agent-scope loads.  The grid is at most one workgroup per CU, so every
+// workgroup is resident; every wait is bounded and raises the shared error
+// flag when it runs out, so the kernel always ends.  Same row arithmetic as
+// the level launches: bitwise identical.
 #include <algorithm>
 
 #include "lap_core.hpp"
@@ -131,12 +143,84 @@ __global__ __launch_bounds__(kBlock) void k_tri_syncfree(
   }
 }
 
-int *sf_state() {  // [ticket, error]
+// state[2]: barrier arrivals (reset before every solve)
+__global__ void k_tb_reset(int *state, const int *__restrict__ done) {
+  if (done && *done) return;
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&state[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr unsigned kTbSpinLimit = 1u << 22;
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_tri_barrier(
+    int nlevels, const int *__restrict__ level_ptr, const int *__restrict__ rows,
+    const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
+    const double *__restrict__ diag, const double *__restrict__ x, double *y, int *state,
+    const int *__restrict__ done) {
+  if (done && *done) return;
+  __shared__ int failed;
+  const int G = gridDim.x;
+  for (int l = 0; l < nlevels; ++l) {
+    const int l0 = level_ptr[l], l1 = level_ptr[l + 1];
+    for (int t = l0 + blockIdx.x * BS + (int)threadIdx.x; t < l1; t += G * BS) {
+      const int row = rows[t];
+      double acc = x[row];
+      double piv = 0.0;
+      bool in_row = false;
+      const int e1 = ia[row + 1];
+      for (int e = ia[row]; e < e1; ++e) {
+        const int c = ja[e];
+        if (c == row) {
+          piv = a[e];
+          in_row = true;
+          continue;
+        }
+        acc -= a[e] * sf_load(y + c);
+      }
+      if (!in_row && diag) piv = diag[row];
+      const double r = acc / piv;
+      __hip_atomic_store(reinterpret_cast<unsigned long long *>(y + row),
+                         (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (l + 1 == nlevels) break;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
+    __syncthreads();
+    if (threadIdx.x < kWave) {  // wave 0: arrive, then wait for every workgroup
+      if (threadIdx.x == 0) atomicAdd(&state[2], 1);
+      const int target = (l + 1) * G;
+      unsigned spins = 0;
+      int ok = 1;
+      for (;;) {
+        const int cur = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&state[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const int err = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (cur >= target) break;
+        if (err || ++spins > kTbSpinLimit) {
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      if (threadIdx.x == 0) {
+        if (!ok)
+          __hip_atomic_store(&state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        failed = !ok;
+      }
+    }
+    __syncthreads();
+    if (failed) break;
+  }
+}
+
+int *sf_state() {  // [ticket, error, barrier arrivals, pad]
   static DevBufPtr buf;
   if (!buf) {
-    buf = dev_alloc(2 * sizeof(int));
+    buf = dev_alloc(4 * sizeof(int));
     if (buf) {
-      LAP_HIP(hipMemsetAsync(buf->p, 0, 2 * sizeof(int), stream()));
+      LAP_HIP(hipMemsetAsync(buf->p, 0, 4 * sizeof(int), stream()));
       sync();
     }
   }
@@ -204,6 +288,7 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
   for (int l = 0; l < nlev; ++l)
     p->max_level_rows = std::max(p->max_level_rows, p->level_ptr[l + 1] - p->level_ptr[l]);
   p->order = dev_upload(order);
+  p->level_ptr_dev = dev_upload(p->level_ptr);
   // sync-free chunks: <= kSfChunk rows, never spanning two levels
   std::vector<int> cp;
   cp.reserve((size_t)n / kSfChunk + (size_t)nlev + 1);
@@ -247,13 +332,32 @@ int tri_error() {
 
 void tri_clear_error() {
   int *st = sf_state();
-  if (st) LAP_HIP(hipMemsetAsync(st, 0, 2 * sizeof(int), stream()));
+  if (st) LAP_HIP(hipMemsetAsync(st, 0, 4 * sizeof(int), stream()));
 }
 
 void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
                       const double *diag, const double *x, double *y, const int *done) {
   const int *order = p.order->as<int>();
   // in place (x == y) needs x intact while y is filled: level launches then
+  if (options().sptrsv_mode == 2 && x != y && p.nlevels > 0) {
+    int *st = sf_state();
+    if (st) {
+      k_tb_reset<<<1, kWave, 0, stream()>>>(st, done);
+      const int *lp = p.level_ptr_dev->as<int>();
+      if (options().tb_block == 1024) {  // fewer, larger workgroups: fewer arrivals
+        const unsigned g = (unsigned)std::max(
+            1, std::min((p.max_level_rows + 1023) / 1024, std::max(1, device_cus() / 4)));
+        k_tri_barrier<1024><<<g, 1024, 0, stream()>>>(p.nlevels, lp, order, ia, ja, a, diag, x,
+                                                      y, st, done);
+      } else {
+        const unsigned g = (unsigned)std::max(
+            1, std::min((p.max_level_rows + kBlock - 1) / kBlock, device_cus()));
+        k_tri_barrier<kBlock><<<g, kBlock, 0, stream()>>>(p.nlevels, lp, order, ia, ja, a, diag,
+                                                          x, y, st, done);
+      }
+      return;
+    }
+  }
   int *st = (options().sptrsv_mode == 1 && x != y) ? sf_state() : nullptr;
   if (st && p.n > 0) {
     const unsigned gf = grid_for((size_t)p.n, 4, kBlock, 2048);
@@ -299,7 +403,7 @@ static void tri_entry(int n, int nnz, const int *ia, const int *ja, const double
   }
   tri_solve_launch(*p, ia, ja, a, dg, x, y, nullptr);
   sync();
-  if (options().sptrsv_mode == 1 && tri_error()) {
+  if (options().sptrsv_mode != 0 && tri_error()) {
     set_error(LAP_ERR_NUMERIC, "triangular solve: a dependency wait ran out (result invalid)");
     tri_clear_error();
   }

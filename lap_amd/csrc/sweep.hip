@@ -112,7 +112,9 @@ __global__ __launch_bounds__(BS) void k_sweep_pair(
   __shared__ double prod[TILE];
   __shared__ double red[BS / kWave];
   const BlockWalk w = block_walk(nblk, xcd);
-  for (int b = w.begin; b < w.end; b += w.step) {
+  for (int s = w.begin; s < w.end; s += w.step) {
+    const int b = w.block(s);
+    if (b >= nblk) continue;
     const int r0 = rowblk[b], r1 = rowblk[b + 1];
     const int l0 = lia[r0], l1 = lia[r1], u0 = uia[r0], u1 = uia[r1];
     const int cl = l1 - l0, cu = u1 - u0;

@@ -63,7 +63,8 @@ def test_spmv_long_rows(orc):
 
 @pytest.mark.parametrize("rows,walk,variant", [(64, 1, 0), (512, 2, 0), (256, 0, 1), (1, 1, 2),
                                                (256, 2, 3), (128, 0, 4), (256, 1, 5), (256, 0, 6),
-                                               (512, 2, 6)])
+                                               (512, 2, 6), (256, 3, 0), (64, 4, 6), (1, 5, 2),
+                                               (128, 3, 4), (16, 5, 5)])
 def test_spmv_blocking_invariant(orc, rows, walk, variant):
     """Row-block size, walk order and body variant change scheduling, never
     the bits."""
@@ -244,11 +245,12 @@ def test_entry_points_refuse_host_vectors():
         L.check()
 
 
-@pytest.fixture
-def sync_free():
-    """Triangular solves in sync-free mode (one launch, dependency waits on
-    the values themselves) for the duration of a test."""
-    prev = S.set_option("sptrsv_mode", 1)
+@pytest.fixture(params=[1, 2], ids=["syncfree", "levelbarrier"])
+def sync_free(request):
+    """Triangular solves in one launch for the duration of a test: mode 1
+    (sync-free, dependency waits on the values themselves) or mode 2 (one
+    grid barrier per level)."""
+    prev = S.set_option("sptrsv_mode", request.param)
     yield
     S.set_option("sptrsv_mode", prev)
 

@@ -262,12 +262,14 @@ def test_jacobi_variants_are_bitwise_identical(pname):
         S.set_option("jacobi_variant", prev)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("kind", ["ichol", "GS_std"])
 @pytest.mark.parametrize("pname", ["grid3d_16", "graph_big"])
-def test_sync_free_trisolve_pcg_is_bitwise_the_level_scheduled_one(kind, pname):
-    """sptrsv_mode 1 (one sync-free launch per triangular solve) gives the
-    level-scheduled solve's bits, so whole PCG runs agree exactly."""
-    print(f"[case] sync-free pcg kind={kind} problem={pname}", flush=True)
+def test_sync_free_trisolve_pcg_is_bitwise_the_level_scheduled_one(kind, pname, mode):
+    """sptrsv_mode 1 (one sync-free launch per triangular solve) and 2 (one
+    launch, a grid barrier per level) give the level-scheduled solve's bits,
+    so whole PCG runs agree exactly."""
+    print(f"[case] one-launch trisolve pcg mode={mode} kind={kind} problem={pname}", flush=True)
     Ah = PROBLEMS[pname]()
     A = S.DeviceCSR.from_csr(Ah)
     P = S.make_preconditioner(A, Ah, kind)
@@ -277,7 +279,7 @@ def test_sync_free_trisolve_pcg_is_bitwise_the_level_scheduled_one(kind, pname):
         for tol, maxit in ((1e-30, 20), (1e-10, 5000)):
             S.set_option("sptrsv_mode", 0)
             r0 = S.pcg(A, b, P, tol=tol, maxit=maxit)
-            S.set_option("sptrsv_mode", 1)
+            S.set_option("sptrsv_mode", mode)
             r1 = S.pcg(A, b, P, tol=tol, maxit=maxit)
             assert (r1.iterations, r1.flag) == (r0.iterations, r0.flag)
             assert np.array_equal(r1.history, r0.history)
