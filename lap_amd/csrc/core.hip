@@ -163,7 +163,6 @@ int lap_set_option(const char *name, int value) {
   else if (!strcmp(name, "use_graph")) slot = &o.use_graph;
   else if (!strcmp(name, "jacobi_variant")) slot = &o.jacobi_variant;
   else if (!strcmp(name, "fuse_sweeps")) slot = &o.fuse_sweeps;
-  else if (!strcmp(name, "tb_block")) slot = &o.tb_block;
   else if (!strcmp(name, "sptrsv_mode")) slot = &o.sptrsv_mode;
   else if (!strcmp(name, "spmm_wg_per_cu")) slot = &o.spmm_wg_per_cu;
   else if (!strcmp(name, "sf_wg_per_cu")) slot = &o.sf_wg_per_cu;
@@ -185,10 +184,6 @@ int lap_set_option(const char *name, int value) {
   }
   if (!strcmp(name, "sptrsv_mode") && (value < 0 || value > 2)) {
     set_error(LAP_ERR_ARG, "sptrsv_mode must be 0, 1 or 2");
-    return *slot;
-  }
-  if (!strcmp(name, "tb_block") && value != 256 && value != 1024) {
-    set_error(LAP_ERR_ARG, "tb_block must be 256 or 1024");
     return *slot;
   }
   if (!strcmp(name, "fuse_sweeps") && (value < 0 || value > 1)) {

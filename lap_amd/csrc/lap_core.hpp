@@ -206,7 +206,6 @@ struct Options {
                            // 1 one sync-free launch, 2 one launch with a
                            // grid barrier per level
   int sf_wg_per_cu = 8;    // sync-free solve: workgroups per CU
-  int tb_block = 256;      // level-barrier solve: workgroup size (256 or 1024)
   int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
   int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;

@@ -343,18 +343,10 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
     int *st = sf_state();
     if (st) {
       k_tb_reset<<<1, kWave, 0, stream()>>>(st, done);
-      const int *lp = p.level_ptr_dev->as<int>();
-      if (options().tb_block == 1024) {  // fewer, larger workgroups: fewer arrivals
-        const unsigned g = (unsigned)std::max(
-            1, std::min((p.max_level_rows + 1023) / 1024, std::max(1, device_cus() / 4)));
-        k_tri_barrier<1024><<<g, 1024, 0, stream()>>>(p.nlevels, lp, order, ia, ja, a, diag, x,
-                                                      y, st, done);
-      } else {
-        const unsigned g = (unsigned)std::max(
-            1, std::min((p.max_level_rows + kBlock - 1) / kBlock, device_cus()));
-        k_tri_barrier<kBlock><<<g, kBlock, 0, stream()>>>(p.nlevels, lp, order, ia, ja, a, diag,
-                                                          x, y, st, done);
-      }
+      const unsigned g = (unsigned)std::max(
+          1, std::min((p.max_level_rows + kBlock - 1) / kBlock, device_cus()));
+      k_tri_barrier<kBlock><<<g, kBlock, 0, stream()>>>(p.nlevels, p.level_ptr_dev->as<int>(),
+                                                        order, ia, ja, a, diag, x, y, st, done);
       return;
     }
   }
