@@ -266,6 +266,40 @@ def bench_energy(S, torch, A, b, P, tol: float, solves: int = 3) -> dict:
     return out
 
 
+def bench_value_indexed(S, torch, A, b, P, tol: float, ref, reps: int) -> dict:
+    """The headline solve with the SpMV reading value-indexed CSR (option
+    spmv_format 1: one-byte indices into the matrix's 5 distinct doubles,
+    5 B per nonzero instead of 12).  Iterates are bitwise the CSR solve's.
+    Its SpMV kernel is timed live; its roofline uses its own algorithmic
+    bytes, 5*nnz + 4(n+1) + 16n."""
+    try:
+        S.set_option("spmv_format", 1)
+        S.pcg(A, b, P, tol=tol, maxit=25000)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = S.pcg(A, b, P, tol=tol, maxit=25000)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        x = torch.randn(A.n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        ms = S.spmv_timed(A, x, y, reps, fused=True)
+    finally:
+        S.set_option("spmv_format", 0)
+    vib = 5 * A.nnz + 4 * (A.n + 1) + 16 * A.n
+    ach = vib / (ms * 1e-3) / 1e9
+    return {
+        "value": wall, "unit": "s", "iterations": r.iterations, "flag": r.flag,
+        "device_ms_per_solve": r.gpu_ms,
+        "ms_per_iteration": r.gpu_ms / max(r.iterations, 1),
+        "bitwise_equal_to_csr_solve": bool(torch.equal(r.x, ref.x)
+                                           and r.iterations == ref.iterations),
+        "speedup_vs_csr": ref.gpu_ms / r.gpu_ms,
+        "spmv_roofline": {"bound": "hbm", "kernel": "k_spmv_pq_vi", "achieved": ach,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                          "algorithmic_bytes": vib, "avg_ms": ms},
+    }
+
+
 def bench_adjoint(torch, np, n: int = 1_000_000, tol: float = 1e-10) -> dict:
     """Forward + adjoint solve of L_w x = b on a random geometric graph (n
     points U[0,1]^2, r = sqrt(10/(pi n)), w = exp(-d^2/sigma^2), sigma = r/2)
@@ -590,6 +624,9 @@ def main() -> int:
     if not args.no_extras:
         for name, fn in (("preconditioners",
                           lambda: bench_preconditioners(S, torch, A, Ah, b, args.tol)),
+                         ("value_indexed",
+                          lambda: bench_value_indexed(S, torch, A, b, P, args.tol, results[-1],
+                                                      args.spmv_reps)),
                          ("energy", lambda: bench_energy(S, torch, A, b, P, args.tol)),
                          ("sweeps", lambda: bench_sweeps(S, torch, A, Ah, b, args.tol)),
                          ("block", lambda: bench_block(S, torch, np)),

@@ -208,6 +208,8 @@ struct Options {
   int sf_wg_per_cu = 8;    // sync-free solve: workgroups per CU
   int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
   int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
+  int spmv_format = 0;     // PCG SpMV: 0 CSR values, 1 value-indexed when the
+                           // matrix has at most 255 distinct values
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
                            // 1 three kernels, x in K_C; 2 two kernels (p in the
                            // SpMV gather)
@@ -242,6 +244,12 @@ void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *
                  const int *done = nullptr);
 // Persistent SpMV grid: min(row blocks, 8 workgroups per CU).
 unsigned spmv_grid(const SpmvPlan &p);
+
+// ------------------------------------------------- value-indexed CSR
+// Encodes a[0..nnz) as one-byte indices into a table of its distinct values
+// (at most 255, bit patterns ascending).  Returns the table size, 0 when the
+// values do not fit, -1 after an error.
+int vi_encode(long long nnz, const double *a, unsigned char *vi, double *dict);
 
 // --------------------------------------------------- SpMV-only sweeps
 // Row blocks over a pair of CSR matrices (the strict L and U of a split),

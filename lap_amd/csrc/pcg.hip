@@ -187,6 +187,67 @@ __global__ __launch_bounds__(BS) void k_spmv_pq(int nblk, const int *__restrict_
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// K_A over value-indexed CSR: the same kernel with the values read as
+// dict[vi[e]], the table staged in LDS.
+template <int BS, int TILE>
+__global__ __launch_bounds__(BS) void k_spmv_pq_vi(int nblk, const int *__restrict__ rowblk,
+                                                   const int *__restrict__ ia,
+                                                   const int *__restrict__ ja,
+                                                   const unsigned char *__restrict__ vi,
+                                                   const double *__restrict__ dict, int nd,
+                                                   const double *__restrict__ p,
+                                                   double *__restrict__ q, PcgCtl *ctl,
+                                                   double *partials, int xcd) {
+  if (ctl->done) return;
+  if (ctl->pend_done) {  // the previous kernel stopped the solve: commit it
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = 1;
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->pend_valid) {
+    ctl->iter = ctl->pend_iter;
+    ctl->rho = ctl->pend_rho;
+    ctl->pend_valid = 0;
+  }
+  __shared__ double prod[TILE];
+  __shared__ double red[BS / kWave];
+  __shared__ double table[256];
+  for (int i = threadIdx.x; i < nd; i += BS) table[i] = dict[i];
+  __syncthreads();
+  PqEpi epi{q, p, 0.0};
+  const BlockWalk w = block_walk(nblk, xcd);
+  for (int s = w.begin; s < w.end; s += w.step) {
+    const int b = w.block(s);
+    if (b >= nblk) continue;
+    spmv_block_rows_vi<BS, TILE>(b, rowblk, ia, ja, vi, table, p, 1.0, prod, red, epi);
+  }
+  const double t = block_sum<BS>(epi.acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// A value-indexed copy of the matrix values for one solve (vi == nullptr:
+// the CSR values are used).
+struct ViVals {
+  const unsigned char *vi = nullptr;
+  const double *dict = nullptr;
+  int nd = 0;
+};
+
+// Encodes a when options().spmv_format asks for it and the values fit.
+static ViVals vi_prepare(int nnz, const double *a) {
+  ViVals v;
+  if (options().spmv_format != 1) return v;
+  unsigned char *vi =
+      reinterpret_cast<unsigned char *>(scratch_vec("vi_idx", ((size_t)nnz + 7) / 8 + 1));
+  double *dict = scratch_vec("vi_dict", 256);
+  const int nd = vi_encode(nnz, a, vi, dict);
+  if (nd > 0) {
+    v.vi = vi;
+    v.dict = dict;
+    v.nd = nd;
+  }
+  return v;
+}
+
 // Jacobi K_B: alpha = rho/(p.q); x += alpha p; r -= alpha q; partials of
 // r.w (w = d_r r) and r.r
 template <int BS>
@@ -578,9 +639,15 @@ static void launch_init(unsigned g, const SpmvPlan &A, const int *ia, const int 
 
 static void launch_spmv_pq(unsigned g, const SpmvPlan &A, const int *ia, const int *ja,
                            const double *a, const double *p, double *q, PcgCtl *ctl,
-                           double *part) {
+                           double *part, const ViVals &vv = ViVals()) {
   const int xcd = options().spmv_walk;
   const int *rb = A.rowblk->as<int>();
+  if (vv.vi) {
+    k_spmv_pq_vi<kBlock, kSpmvTile><<<g, kBlock, 0, stream()>>>(A.nblocks, rb, ia, ja, vv.vi,
+                                                                vv.dict, vv.nd, p, q, ctl, part,
+                                                                xcd);
+    return;
+  }
 #define LAP_CASE(V)                                                                          \
   case V:                                                                                    \
     k_spmv_pq<kBlock, kSpmvTile, V><<<g, kBlock, 0, stream()>>>(A.nblocks, rb, ia, ja, a, p, q, \
@@ -936,6 +1003,7 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   launch_init(gA, A, ia, ja, a, x, b, r, R0);
   const bool jac = o.kind == P_JACOBI;
   const int jv = jac ? options().jacobi_variant : 0;
+  const ViVals vv = jv == 2 ? ViVals() : vi_prepare(nnz, a);
   const bool fused = jv == 2, xin_c = jv == 1;
   // fused Jacobi: p alternates between two buffers (p_t is read while
   // p_{t+1} is written); p_1 goes to pp[1]
@@ -963,12 +1031,12 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
       k_jac_rupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA, R3);
       ++s_iter;
     } else if (xin_c) {
-      launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2);
+      launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2, vv);
       k_jac_update_nx<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA,
                                                            R3);
       k_jac_px<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, r, pd->d_r, p, ctl, hist, R3, (int)gV);
     } else if (jac) {
-      launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2);
+      launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2, vv);
       k_jac_update<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, r, p, q, pd->d_r, ctl, R2,
                                                         (int)gA, R3);
       k_jac_p<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, pd->d_r, p, ctl, hist, R3, (int)gV);
@@ -976,7 +1044,7 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
       prec_launch(o, r, w, done);
       k_rw<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, w, ctl, R1);
       k_pupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, w, p, ctl, R1, (int)gV);
-      launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2);
+      launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2, vv);
       k_update<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, r, p, q, ctl, R2, (int)gA, R3);
       k_conv<kBlock><<<1, kBlock, 0, stream()>>>(ctl, hist, R3, (int)gV);
     }
@@ -1146,7 +1214,8 @@ double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const do
   PcgCtl *ctl = reinterpret_cast<PcgCtl *>(scratch_vec("timed_ctl", sizeof(PcgCtl) / 8 + 1));
   LAP_HIP(hipMemcpyAsync(ctl, &h, sizeof h, hipMemcpyHostToDevice, stream()));
   const unsigned g = spmv_grid(*A);
-  auto launch = [&]() { launch_spmv_pq(g, *A, ia, ja, a, p, q, ctl, ws.partials + kR2); };
+  const ViVals vv = vi_prepare(nnz, a);  // the kernel the solve would run
+  auto launch = [&]() { launch_spmv_pq(g, *A, ia, ja, a, p, q, ctl, ws.partials + kR2, vv); };
   launch();
   hipEvent_t e0, e1;
   LAP_HIP(hipEventCreate(&e0));
