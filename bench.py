@@ -496,6 +496,9 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the preconditioner, multi-RHS, batched, adjoint and input-path "
                          "measurements")
+    ap.add_argument("--only", default="",
+                    help="comma-separated extras to run (preconditioners, value_indexed, energy, "
+                         "sweeps, block, batched, adjoint, input); default all")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the kernels the HBM-traffic counters are read on: a 1 GiB "
                          "device copy (calibration) and the fused SpMV, then exit")
@@ -633,6 +636,8 @@ def main() -> int:
                          ("batched", lambda: bench_batched(S, torch, np)),
                          ("adjoint", lambda: bench_adjoint(torch, np)),
                          ("input", lambda: bench_input(S, torch, np, A, Ah, args.size))):
+            if args.only and name not in args.only.split(","):
+                continue
             t0 = time.perf_counter()
             try:
                 out[name] = fn()
@@ -645,7 +650,7 @@ def main() -> int:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(Ah, b_h, args.prec, args.tol, its, args.cpu_iters)
         log(f"cpu baseline in {time.perf_counter() - t0:.1f}s: {out['cpu_baseline']['value']:.1f} s")
-        if not args.no_extras:
+        if not args.no_extras and "input" in out:
             t0 = time.perf_counter()
             try:
                 out["input"]["cpu_baseline"] = cpu_input_baseline(np)
