@@ -505,8 +505,7 @@ def main() -> int:
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the solve would shard)", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): the solver has no multi-GPU path", file=sys.stderr)
         return 2
 
     import numpy as np
