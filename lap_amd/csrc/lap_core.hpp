@@ -279,6 +279,7 @@ struct TriPlan {
   int nlevels = 0;
   std::vector<int> level_ptr;  // host, nlevels + 1
   DevBufPtr level_ptr_dev;     // the same on the device (level-barrier mode)
+  DevBufPtr desc;              // int4[n] in level order: {row, ia[row], ia[row+1], 0}
   DevBufPtr order;             // int[n]: rows sorted by level
   int max_level_rows = 0;
   bool has_diag = true;        // every row holds its pivot

@@ -44,19 +44,22 @@ namespace lap {
 
 namespace {
 
+// desc[t] = {row, first nonzero, end}: one load gives the row and its range
+// (the row pointers are not read in the solve).
 __global__ __launch_bounds__(kBlock) void k_tri_level(
-    int cnt, const int *__restrict__ rows, const int *__restrict__ ia,
-    const int *__restrict__ ja, const double *__restrict__ a, const double *__restrict__ diag,
-    const double *x, double *y, const int *__restrict__ done) {
+    int cnt, const int4 *__restrict__ desc, const int *__restrict__ ja,
+    const double *__restrict__ a, const double *__restrict__ diag, const double *x, double *y,
+    const int *__restrict__ done) {
   if (done && *done) return;
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= cnt) return;
-  const int row = rows[t];
+  const int4 d = desc[t];
+  const int row = d.x;
   double acc = x[row];
   double piv = 0.0;
   bool in_row = false;
-  const int e1 = ia[row + 1];
-  for (int e = ia[row]; e < e1; ++e) {
+  const int e1 = d.z;
+  for (int e = d.y; e < e1; ++e) {
     const int c = ja[e];
     if (c == row) {
       piv = a[e];
@@ -289,6 +292,14 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
     p->max_level_rows = std::max(p->max_level_rows, p->level_ptr[l + 1] - p->level_ptr[l]);
   p->order = dev_upload(order);
   p->level_ptr_dev = dev_upload(p->level_ptr);
+  std::vector<int> desc((size_t)n * 4);
+  for (int t = 0; t < n; ++t) {
+    desc[(size_t)t * 4] = order[(size_t)t];
+    desc[(size_t)t * 4 + 1] = ia[order[(size_t)t]];
+    desc[(size_t)t * 4 + 2] = ia[order[(size_t)t] + 1];
+    desc[(size_t)t * 4 + 3] = 0;
+  }
+  p->desc = dev_upload(desc);
   // sync-free chunks: <= kSfChunk rows, never spanning two levels
   std::vector<int> cp;
   cp.reserve((size_t)n / kSfChunk + (size_t)nlev + 1);
@@ -362,11 +373,11 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
                                                std::max(1, options().sf_naps));
     return;
   }
+  const int4 *desc = p.desc->as<int4>();
   for (int l = 0; l < p.nlevels; ++l) {
     const int cnt = p.level_ptr[l + 1] - p.level_ptr[l];
     const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock);
-    k_tri_level<<<g, kBlock, 0, stream()>>>(cnt, order + p.level_ptr[l], ia, ja, a, diag, x, y,
-                                            done);
+    k_tri_level<<<g, kBlock, 0, stream()>>>(cnt, desc + p.level_ptr[l], ja, a, diag, x, y, done);
   }
 }
 
