@@ -271,7 +271,8 @@ def bench_value_indexed(S, torch, A, b, P, tol: float, ref, reps: int) -> dict:
     spmv_format 1: one-byte indices into the matrix's 5 distinct doubles,
     5 B per nonzero instead of 12).  Iterates are bitwise the CSR solve's.
     Its SpMV kernel is timed live; its roofline uses its own algorithmic
-    bytes, 5*nnz + 4(n+1) + 16n."""
+    bytes, 5*nnz + 4(n+1) + 16n.  The mixed-precision solve with its FP32
+    off-diagonal copy value-indexed is timed too, against both FP64 solves."""
     try:
         S.set_option("spmv_format", 1)
         S.pcg(A, b, P, tol=tol, maxit=25000)  # warm
@@ -283,6 +284,12 @@ def bench_value_indexed(S, torch, A, b, P, tol: float, ref, reps: int) -> dict:
         x = torch.randn(A.n, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
         ms = S.spmv_timed(A, x, y, reps, fused=True)
+        S.pcg_mixed(A, b, P, tol=tol, maxit=25000)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m = S.pcg_mixed(A, b, P, tol=tol, maxit=25000)
+        torch.cuda.synchronize()
+        mwall = time.perf_counter() - t0
     finally:
         S.set_option("spmv_format", 0)
     vib = 5 * A.nnz + 4 * (A.n + 1) + 16 * A.n
@@ -294,6 +301,14 @@ def bench_value_indexed(S, torch, A, b, P, tol: float, ref, reps: int) -> dict:
         "bitwise_equal_to_csr_solve": bool(torch.equal(r.x, ref.x)
                                            and r.iterations == ref.iterations),
         "speedup_vs_csr": ref.gpu_ms / r.gpu_ms,
+        "mixed_precision": {
+            "value": mwall, "unit": "s", "iterations": m.iterations,
+            "refinements": m.refinements, "flag": m.flag, "device_ms_per_solve": m.gpu_ms,
+            "true_rel_res": m.true_res / m.res0,
+            "speedup_vs_fp64_csr": ref.gpu_ms / m.gpu_ms,
+            "speedup_vs_fp64_value_indexed": r.gpu_ms / m.gpu_ms,
+            "scheme": "the mixed_precision solve with its FP32 off-diagonal copy value-indexed",
+        },
         "spmv_roofline": {"bound": "hbm", "kernel": "k_spmv_pq_vi", "achieved": ach,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                           "algorithmic_bytes": vib, "avg_ms": ms},

@@ -182,6 +182,43 @@ __global__ __launch_bounds__(BS) void k_mx_spmv_pq(int nblk, const int *__restri
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// The same over the value-indexed copy: off-diagonal value = dict[vi[e]],
+// the table staged in LDS; bitwise the same products and sums.
+template <int BS, int TILE>
+__global__ __launch_bounds__(BS) void k_mx_spmv_pq_vi(int nblk, const int *__restrict__ rowblk,
+                                                      const int *__restrict__ ia,
+                                                      const int *__restrict__ ja,
+                                                      const unsigned char *__restrict__ vi,
+                                                      const float *__restrict__ dict, int nd,
+                                                      const double *__restrict__ diag,
+                                                      const float *__restrict__ p,
+                                                      float *__restrict__ q, MixCtl *ctl,
+                                                      double *part, int walk) {
+  if (ctl->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->pend_valid) {
+    ctl->iter = ctl->pend_iter;
+    ctl->rho = ctl->pend_rho;
+    ctl->maxr2 = ctl->pend_maxr2;
+    ctl->refinements = ctl->pend_refinements;
+    ctl->pend_valid = 0;
+  }
+  __shared__ double prod[TILE];
+  __shared__ double red[BS / kWave];
+  __shared__ float table[256];
+  for (int i = threadIdx.x; i < nd; i += BS) table[i] = dict[i];
+  __syncthreads();
+  Pq32Epi epi{q, p, diag, 0.0};
+  const BlockWalk w = block_walk(nblk, walk);
+  for (int s = w.begin; s < w.end; s += w.step) {
+    const int blk = w.block(s);
+    if (blk >= nblk) continue;
+    spmv_block_rows_vi<BS, TILE, Pq32Epi, float, float>(blk, rowblk, ia, ja, vi, table, p, 1.0,
+                                                        prod, red, epi);
+  }
+  const double t = block_sum<BS>(epi.acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // alpha = rho/(p.q); x32 += alpha p; r32 -= alpha q (each rounded once from
 // FP64); partials of r.r and, with Jacobi, r.w for w = d_r r (stride 2:
 // [rz, rr]).
@@ -421,6 +458,10 @@ __global__ __launch_bounds__(kBlock) void k_copy_f32(int n, const float *__restr
 struct MixPlan {
   int n = 0, nnz_off = 0;
   DevBufPtr ia_o, ja_o, a32_o, diag;
+  // value-indexed copy of a32_o (when it has at most 255 distinct values):
+  // one byte per off-diagonal entry and the table of the values
+  DevBufPtr vi_o, dict_o;
+  int nd = 0;
   const SpmvPlan *off = nullptr;
   std::map<const double *, DevBufPtr> dr32;  // by d_r pointer
   std::map<const IcPlan *, std::pair<DevBufPtr, DevBufPtr>> ic32;
@@ -459,6 +500,24 @@ MixPlan *mix_plan(int n, int nnz, const int *ia, const int *ja, const double *a)
   P->ja_o = dev_upload(jo);
   P->a32_o = dev_upload(ao);
   P->diag = dev_upload(dg);
+  {  // the distinct FP32 values, ascending by bit pattern
+    std::vector<unsigned> bitsv(ao.size());
+    std::memcpy(bitsv.data(), ao.data(), ao.size() * sizeof(float));
+    std::vector<unsigned> keys(bitsv);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    if (!keys.empty() && keys.size() <= 255) {
+      std::vector<unsigned char> vi(ao.size());
+      for (size_t e = 0; e < ao.size(); ++e)
+        vi[e] = (unsigned char)(std::lower_bound(keys.begin(), keys.end(), bitsv[e]) -
+                                keys.begin());
+      std::vector<float> dict(keys.size());
+      std::memcpy(dict.data(), keys.data(), keys.size() * sizeof(float));
+      P->vi_o = dev_upload(vi);
+      P->dict_o = dev_upload(dict);
+      P->nd = (int)keys.size();
+    }
+  }
   P->off = spmv_plan(n, P->nnz_off, P->ia_o->as<int>(), P->ja_o->as<int>());
   if (!P->off) return nullptr;
   MixPlan *raw = P.get();
@@ -591,11 +650,21 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
           cnt, order + T.level_ptr[l], tia, tja, ta, in, outv, done);
     }
   };
-  auto iteration = [&]() {
-    if (jac) {
+  // value-indexed inner SpMV when spmv_format asks for it and the table fits
+  const bool vi32 = options().spmv_format == 1 && M->nd > 0;
+  auto spmv32 = [&]() {
+    if (vi32)
+      k_mx_spmv_pq_vi<kBlock, kSpmvTile><<<gO, kBlock, 0, stream()>>>(
+          M->off->nblocks, rb_o, io, jo, M->vi_o->as<unsigned char>(), M->dict_o->as<float>(),
+          M->nd, dg, p32, q32, ctl, Q1, walk);
+    else
       k_mx_spmv_pq<kBlock, kSpmvTile><<<gO, kBlock, 0, stream()>>>(M->off->nblocks, rb_o, io,
                                                                    jo, ao, dg, p32, q32, ctl,
                                                                    Q1, walk);
+  };
+  auto iteration = [&]() {
+    if (jac) {
+      spmv32();
       k_mx_update<kBlock, true><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, d_r32, ctl,
                                                              Q1, (int)gO, Q2);
       k_mx_fold<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
@@ -612,9 +681,7 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
       }
       k_mx_rw<kBlock><<<gV, kBlock, 0, stream()>>>(n, r32, w32, ctl, Q0);
       k_mx_pupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, w32, p32, ctl, Q0, (int)gV);
-      k_mx_spmv_pq<kBlock, kSpmvTile><<<gO, kBlock, 0, stream()>>>(M->off->nblocks, rb_o, io,
-                                                                   jo, ao, dg, p32, q32, ctl,
-                                                                   Q1, walk);
+      spmv32();
       k_mx_update<kBlock, false><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, nullptr,
                                                               ctl, Q1, (int)gO, Q2);
       k_mx_fold<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);

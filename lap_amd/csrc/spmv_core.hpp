@@ -126,22 +126,23 @@ __device__ __forceinline__ void spmv_block_rows_pf(int b, const int *__restrict_
 }
 
 // Variant 0 body over value-indexed CSR: the value of nonzero e is
-// dict[vi[e]] (dict in LDS).  Same products and row order as
-// spmv_block_rows.
-template <int BS, int TILE, class Epi>
+// dict[vi[e]] (dict in LDS; VT double, or float for the mixed-precision
+// copy).  Same products and row order as spmv_block_rows.
+template <int BS, int TILE, class Epi, class VT = double, class XT = double>
 __device__ __forceinline__ void spmv_block_rows_vi(int b, const int *__restrict__ rowblk,
                                                    const int *__restrict__ ia,
                                                    const int *__restrict__ ja,
                                                    const unsigned char *__restrict__ vi,
-                                                   const double *dict,
-                                                   const double *__restrict__ x, double alpha,
+                                                   const VT *dict,
+                                                   const XT *__restrict__ x, double alpha,
                                                    double *prod, double *red, Epi &epi) {
   const int r0 = rowblk[b], r1 = rowblk[b + 1];
   const int e0 = ia[r0], e1 = ia[r1];
   const int cnt = e1 - e0;
   if (cnt > TILE) {
     double s = 0.0;
-    for (int e = e0 + (int)threadIdx.x; e < e1; e += BS) s += (alpha * dict[vi[e]]) * x[ja[e]];
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += BS)
+      s += (alpha * (double)dict[vi[e]]) * (double)x[ja[e]];
     const double t = block_sum<BS>(s, red);
     if (threadIdx.x == 0) epi.store(r0, epi.init(r0) + t);
     return;
@@ -149,7 +150,7 @@ __device__ __forceinline__ void spmv_block_rows_vi(int b, const int *__restrict_
 #pragma unroll 4
   for (int k = threadIdx.x; k < cnt; k += BS) {
     const int e = e0 + k;
-    prod[k] = (alpha * dict[vi[e]]) * x[ja[e]];
+    prod[k] = (alpha * (double)dict[vi[e]]) * (double)x[ja[e]];
   }
   __syncthreads();
   spmv_row_sums<BS>(r0, r1, e0, ia, prod, epi);

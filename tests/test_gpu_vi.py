@@ -108,3 +108,22 @@ def test_value_indexed_spmv_timed_kernel():
         S.set_option("spmv_format", 0)
     assert t0 > 0 and t1 > 0
     assert torch.equal(y0, y1)
+
+
+@pytest.mark.parametrize("kind", ["line_jacobi", "ichol"])
+def test_value_indexed_mixed_precision_bitwise(kind):
+    """The mixed-precision inner SpMV over the value-indexed FP32 copy gives
+    the FP32-CSR solve's bits."""
+    Ah = PROBLEMS["grid3d_16"]()
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, kind)
+    b = S.to_dev(rhs_normal(Ah.n, 22))
+    try:
+        S.set_option("spmv_format", 0)
+        m0 = S.pcg_mixed(A, b, P, tol=1e-10, maxit=5000)
+        S.set_option("spmv_format", 1)
+        m1 = S.pcg_mixed(A, b, P, tol=1e-10, maxit=5000)
+    finally:
+        S.set_option("spmv_format", 0)
+    assert (m0.iterations, m0.refinements, m0.flag) == (m1.iterations, m1.refinements, m1.flag)
+    assert np.array_equal(bits(m0.x), bits(m1.x))
