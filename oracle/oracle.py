@@ -349,20 +349,32 @@ class IoUtils:
         return CSR(n, ia, self._arr(M.csr_ja, int(ia[-1]), np.int32),
                    self._arr(M.csr_vals, int(ia[-1]), np.float64))
 
-    def load(self, path: str, adjacency: bool = False, weighted: int | None = None):
+    def load(self, path: str, adjacency: bool = False, weighted: int | None = None,
+             do_split: bool = True):
         """read -> coo_to_csr -> split (or create_L_and_split when weighted is
-        not None).  Returns dict(A, L, U, D, fields) with every array copied."""
+        not None).  Returns dict(A, L, U, D, fields) with every array copied.
+        do_split=False stops after coo_to_csr: the reference's split sizes L
+        and U as nnz - n (io_utils.c:281-287), so a file without every
+        diagonal entry makes it write out of bounds."""
         A, L, U, D = MMatrix(), MMatrix(), MMatrix(), MMatrix()
         fn = self.lib.read_adjacency_file if adjacency else self.lib.read_mm_file
         with _silence_stdout():
             fn(path.encode(), C.byref(A))
             self.lib.coo_to_csr(C.byref(A))
-            if weighted is None:
+            if not do_split:
+                pass
+            elif weighted is None:
                 self.lib.split(C.byref(A), C.byref(L), C.byref(U), C.byref(D))
             else:
                 self.lib.create_L_and_split(C.byref(A), C.byref(L), C.byref(U), C.byref(D),
                                             int(weighted))
         n = A.n
+        if not do_split:
+            res = dict(A=self._csr(A, n, A.nnz_unpacked),
+                       fields=dict(n=A.n, m=A.m, nnz=A.nnz, nnz_unpacked=A.nnz_unpacked,
+                                   symmetric=A.symmetric))
+            self._free(A.csr_ia, A.csr_ja, A.csr_vals)
+            return res
         res = dict(A=self._csr(A, n, A.nnz_unpacked), L=self._csr(L, n, L.nnz),
                    U=self._csr(U, n, U.nnz),
                    D_vals=self._arr(D.csr_vals, n, np.float64),

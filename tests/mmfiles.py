@@ -159,3 +159,28 @@ def rhs_file(d, n, seed=7):
 
 ALL_MATRICES = [grid_symmetric, general_with_duplicates, value_formats, long_rows_symmetric]
 ALL_ADJACENCY = [adjacency_pattern, adjacency_weighted]
+
+
+def edge_cases(d):
+    """Small files at the edges: 1x1, no entries, empty rows, diagonal only,
+    pattern general, and values 0, -0, nan, inf, -inf (kept bit for bit)."""
+    out = {}
+    out["one_by_one"] = _write(os.path.join(d, "e_one.mtx"),
+                               "%%MatrixMarket matrix coordinate real symmetric", 1, 1,
+                               [0], [0], [4.5])
+    out["no_entries"] = _write(os.path.join(d, "e_none.mtx"),
+                               "%%MatrixMarket matrix coordinate real general", 5, 5, [], [], [])
+    out["empty_rows"] = _write(os.path.join(d, "e_rows.mtx"),
+                               "%%MatrixMarket matrix coordinate real general", 9, 9,
+                               [0, 8, 8, 4], [8, 0, 8, 4], [1.0, 2.0, 3.0, 4.0])
+    out["diagonal_only"] = _write(os.path.join(d, "e_diag.mtx"),
+                                  "%%MatrixMarket matrix coordinate real symmetric", 6, 6,
+                                  list(range(6)), list(range(6)), [float(i + 1) for i in range(6)])
+    out["pattern_general"] = _write(os.path.join(d, "e_pat.mtx"),
+                                    "%%MatrixMarket matrix coordinate pattern general", 4, 4,
+                                    [0, 1, 3, 3, 2], [1, 0, 3, 0, 2])
+    out["special_values"] = _write(os.path.join(d, "e_special.mtx"),
+                                   "%%MatrixMarket matrix coordinate real symmetric", 4, 4,
+                                   [0, 1, 2, 3, 3, 2], [0, 0, 1, 3, 1, 2],
+                                   ["0", "-0", "nan", "inf", "-inf", "-0.0"])
+    return out

@@ -129,3 +129,19 @@ def test_device_entry_points_reject_bad_input():
         L.load().lap_coo_to_csr(3, 3, 2, rows.cpu().numpy().ctypes.data, None, None, 0, 0,
                                 None, None, None, C.byref(cnt))
         L.check()
+
+
+def test_edge_case_files_on_the_device(orc, tmp_path):
+    """The same edge files through lap_coo_to_csr and lap_csr_split: bitwise
+    the oracle (NaN and infinities carried as bits)."""
+    for name, path in MF.edge_cases(str(tmp_path)).items():
+        print(f"[case] edge file {name}", flush=True)
+        A, info = IO.load_mm(path)
+        n, m, sym, r, c, v = orc.mm_read(path)
+        Ao = orc.coo_to_csr(n, r, c, v, sym)
+        assert same(host(A), Ao), name
+        for aug in (False, True):
+            Ld, dd, Ud = IO.split(A, augmented=aug)
+            Lo, do, Uo = orc.split(Ao, augmented=aug)
+            assert same(host(Ld), Lo) and same(host(Ud), Uo), (name, aug)
+            assert np.array_equal(dd.cpu().numpy().view(np.int64), do.view(np.int64)), name

@@ -208,3 +208,27 @@ def test_comments_and_blank_lines_between_entries(tmp_path, orc):
                     "1 1 2.5\n% mid\n\n2 2 -1\n")
     info, r, c, v = IO.read_coo(str(path))
     assert r.tolist() == [0, 1] and c.tolist() == [0, 1] and v.tolist() == [2.5, -1.0]
+
+
+def test_edge_case_files_match_reference(tmp_path, ref_io, our_io, orc):
+    """1x1, no entries, empty rows, diagonal only, pattern general and the
+    values 0, -0, nan, +-inf: parser and COO -> CSR bitwise the reference's
+    (the oracle's too); split where the reference's own split is defined."""
+    for name, path in MF.edge_cases(str(tmp_path)).items():
+        info, r, c, v = IO.read_coo(path)
+        n, m, nnz, sym, rr, rc, rv = ref_io.read(path)
+        assert (info.n, info.nnz) == (n, nnz), name
+        assert np.array_equal(r, rr) and np.array_equal(c, rc), name
+        assert np.array_equal(bits(v), bits(rv)), name
+        # the reference's split needs every diagonal entry (it sizes L and U
+        # as nnz - n); the others stop after coo_to_csr
+        full = name in ("one_by_one", "diagonal_only")
+        R, G = ref_io.load(path, do_split=full), our_io.load(path, do_split=full)
+        assert R["fields"] == G["fields"], name
+        for key in (("A", "L", "U") if full else ("A",)):
+            assert csr_equal(R[key], G[key]), (name, key)
+        if full:
+            assert np.array_equal(bits(R["D_vals"]), bits(G["D_vals"])), name
+        on, om, osym, orr, occ, ov = orc.mm_read(path)
+        A = orc.coo_to_csr(on, orr, occ, ov, osym)
+        assert csr_equal(A, R["A"]), name
