@@ -21,11 +21,16 @@ DEV = "cuda"
 
 
 def ptr(t):
-    """Device pointer of a contiguous CUDA tensor (None passes NULL)."""
+    """Device pointer of a contiguous CUDA tensor (None passes NULL).  An
+    empty view of a non-empty buffer passes the buffer's address (torch
+    reports 0 for it), so zero-length arrays still name device memory."""
     if t is None:
         return None
     if not t.is_cuda or not t.is_contiguous():
         raise L.LapError("expected a contiguous CUDA tensor")
+    if t.numel() == 0:
+        base = t.untyped_storage().data_ptr()
+        return C.c_void_p(base + t.storage_offset() * t.element_size() if base else 0)
     return C.c_void_p(t.data_ptr())
 
 
