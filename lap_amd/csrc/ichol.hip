@@ -193,6 +193,8 @@ const IcPlan *ic_factor(int n, int nnzA, const int *ia, const int *ja, double *a
     k_gather<<<gL, kBlock, 0, stream()>>>(nnzL, d_tperm->as<int>(), P.lval->as<double>(),
                                           P.uval->as<double>());
   }
+  tri_ell_build(*P.lplan, P.lja->as<int>(), P.lval->as<double>(), P.lell);
+  tri_ell_build(*P.uplan, P.uja->as<int>(), P.uval->as<double>(), P.uell);
   std::vector<int> h_bad = dev_download(bad->as<int>(), 2);
   if (h_bad[0]) {
     P.ok = false;
@@ -209,9 +211,9 @@ const IcPlan *ic_factor(int n, int nnzA, const int *ia, const int *ja, double *a
 void ic_apply_launch(const IcPlan &p, const double *x, double *y, double *aux,
                      const int *done) {
   tri_solve_launch(*p.lplan, p.lia->as<int>(), p.lja->as<int>(), p.lval->as<double>(), nullptr,
-                   x, aux, done);
+                   x, aux, done, &p.lell);
   tri_solve_launch(*p.uplan, p.uia->as<int>(), p.uja->as<int>(), p.uval->as<double>(), nullptr,
-                   aux, y, done);
+                   aux, y, done, &p.uell);
 }
 
 }  // namespace lap

@@ -280,6 +280,7 @@ struct TriPlan {
   std::vector<int> level_ptr;  // host, nlevels + 1
   DevBufPtr level_ptr_dev;     // the same on the device (level-barrier mode)
   DevBufPtr desc;              // int4[n] in level order: {row, ia[row], ia[row+1], 0}
+  int max_row = 0;             // longest row (entries, pivot included)
   DevBufPtr order;             // int[n]: rows sorted by level
   int max_level_rows = 0;
   bool has_diag = true;        // every row holds its pivot
@@ -287,10 +288,22 @@ struct TriPlan {
   DevBufPtr chunk_ptr;         // int[nchunks + 1], offsets into order
 };
 const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower);
+// A triangle's rows in level order as fixed-width records (w = 4 or 8
+// entries, padded with column -1): ja[t*w + k], a[t*w + k] for the row at
+// level-order position t.  Built for one set of values; w == 0: not built.
+struct TriEll {
+  int w = 0;
+  DevBufPtr ja, a;
+};
+// Packs (ja, a) of the plan's matrix when its rows hold at most 8 entries.
+void tri_ell_build(const TriPlan &p, const int *ja, const double *a, TriEll &e);
 // y = T^{-1} x (level-scheduled); ctl_done (device int*, may be null) makes
 // every launch a no-op once *ctl_done != 0.
+// `ell` (may be null): the packed rows of the same values, used by the
+// level launches when built.
 void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
-                      const double *diag, const double *x, double *y, const int *ctl_done);
+                      const double *diag, const double *x, double *y, const int *ctl_done,
+                      const TriEll *ell = nullptr);
 // Sync-free mode: nonzero once a dependency wait ran out (the result is then
 // invalid); reset by tri_clear_error.
 int tri_error();
@@ -303,6 +316,7 @@ struct IcPlan {
   DevBufPtr lia, lja, lval;  // L (diagonal last)
   DevBufPtr uia, uja, uval;  // U = L^T (diagonal first)
   const TriPlan *lplan = nullptr, *uplan = nullptr;
+  TriEll lell, uell;  // the factors' rows packed in level order
   bool ok = true;
 };
 const IcPlan *ic_plan_find(const double *vals);

@@ -678,6 +678,7 @@ struct PrecOps {
   const PairPlan *LU = nullptr;                               // L and U together
   const TriPlan *Lt = nullptr, *Ut = nullptr;                 // triangular plans
   const double *ldiag = nullptr, *udiag = nullptr;
+  TriEll lell, uell;  // GS_std: the triangles packed for this solve (run_pcg)
   const IcPlan *ic = nullptr;
   double *aux1 = nullptr, *aux2 = nullptr, *aux3 = nullptr;
 };
@@ -885,18 +886,20 @@ static void prec_launch(const PrecOps &o, const double *r, double *w, const int 
       for (int s = 0; s < o.k; ++s) {
         if (s == 0) {
           // w = 0: r - A*0 == r and 0 + L^{-1} r == L^{-1} r, so solve directly
-          tri_solve_launch(*o.Lt, pd->lia, pd->lja, pd->la, o.ldiag, r, w, done);
+          tri_solve_launch(*o.Lt, pd->lia, pd->lja, pd->la, o.ldiag, r, w, done, &o.lell);
         } else {
           launch_copy(n, r, o.aux2, done);
           spmv_launch(*o.A, o.ia, o.ja, o.a, w, o.aux2, -1.0, 1.0, done);
-          tri_solve_launch(*o.Lt, pd->lia, pd->lja, pd->la, o.ldiag, o.aux2, o.aux1, done);
+          tri_solve_launch(*o.Lt, pd->lia, pd->lja, pd->la, o.ldiag, o.aux2, o.aux1, done,
+                           &o.lell);
           launch_axpy(n, 1.0, o.aux1, w, done);
         }
       }
       for (int s = 0; s < o.k; ++s) {
         launch_copy(n, r, o.aux2, done);
         spmv_launch(*o.A, o.ia, o.ja, o.a, w, o.aux2, -1.0, 1.0, done);
-        tri_solve_launch(*o.Ut, pd->uia, pd->uja, pd->ua, o.udiag, o.aux2, o.aux1, done);
+        tri_solve_launch(*o.Ut, pd->uia, pd->uja, pd->ua, o.udiag, o.aux2, o.aux1, done,
+                         &o.uell);
         launch_axpy(n, 1.0, o.aux1, w, done);
       }
       break;
@@ -976,6 +979,10 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
     return LAP_ERR_ARG;
   PrecOps o;
   if (!prec_setup(o, n, nnz, ia, ja, a, pd)) return lap_last_status();
+  if (o.kind == P_GS_STD) {  // packed triangles for the level launches of this solve
+    tri_ell_build(*o.Lt, pd->lja, pd->la, o.lell);
+    tri_ell_build(*o.Ut, pd->uja, pd->ua, o.uell);
+  }
   ReduceWs &ws = reduce_ws();
   double *r = scratch_vec("pcg_r", (size_t)n);
   double *w = scratch_vec("pcg_w", (size_t)n);

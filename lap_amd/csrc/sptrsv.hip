@@ -73,6 +73,68 @@ __global__ __launch_bounds__(kBlock) void k_tri_level(
   y[row] = acc / piv;
 }
 
+// Packed rows: lane t loads its row's w columns and values together (one
+// dependent stage less than through desc), then the y of earlier levels.
+// Same subtraction order and pivot as k_tri_level: bitwise identical.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_ell_pack(int n, const int4 *__restrict__ desc,
+                                                     const int *__restrict__ ja,
+                                                     const double *__restrict__ a,
+                                                     int *__restrict__ eja,
+                                                     double *__restrict__ ea) {
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    const int4 d = desc[t];
+    const int cnt = d.z - d.y;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      eja[(size_t)t * W + k] = k < cnt ? ja[d.y + k] : -1;
+      ea[(size_t)t * W + k] = k < cnt ? a[d.y + k] : 0.0;
+    }
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_tri_level_ell(
+    int cnt, const int *__restrict__ rows, const int *__restrict__ eja,
+    const double *__restrict__ ea, const double *__restrict__ diag, const double *x, double *y,
+    const int *__restrict__ done) {
+  if (done && *done) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  const int row = rows[t];
+  int c[W];
+  double v[W];
+#pragma unroll
+  for (int k = 0; k < W; k += 4) {
+    const int4 q = *reinterpret_cast<const int4 *>(eja + (size_t)t * W + k);
+    c[k] = q.x;
+    c[k + 1] = q.y;
+    c[k + 2] = q.z;
+    c[k + 3] = q.w;
+  }
+#pragma unroll
+  for (int k = 0; k < W; k += 2) {
+    const double2 q = *reinterpret_cast<const double2 *>(ea + (size_t)t * W + k);
+    v[k] = q.x;
+    v[k + 1] = q.y;
+  }
+  double acc = x[row];
+  double piv = 0.0;
+  bool in_row = false;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    if (c[k] < 0) break;
+    if (c[k] == row) {
+      piv = v[k];
+      in_row = true;
+      continue;
+    }
+    acc -= v[k] * y[c[k]];
+  }
+  if (!in_row && diag) piv = diag[row];
+  y[row] = acc / piv;
+}
+
 constexpr unsigned long long kSentinel = 0x7ff44c4150534e54ull;  // sNaN payload
 constexpr int kSfChunk = 64, kSfClaim = 4;
 constexpr unsigned kSfSpinLimit = 1u << 20;
@@ -290,6 +352,7 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
   for (int i = 0; i < n; ++i) order[fill[level[i]]++] = i;
   for (int l = 0; l < nlev; ++l)
     p->max_level_rows = std::max(p->max_level_rows, p->level_ptr[l + 1] - p->level_ptr[l]);
+  for (int i = 0; i < n; ++i) p->max_row = std::max(p->max_row, ia[i + 1] - ia[i]);
   p->order = dev_upload(order);
   p->level_ptr_dev = dev_upload(p->level_ptr);
   std::vector<int> desc((size_t)n * 4);
@@ -346,9 +409,43 @@ void tri_clear_error() {
   if (st) LAP_HIP(hipMemsetAsync(st, 0, 4 * sizeof(int), stream()));
 }
 
+void tri_ell_build(const TriPlan &p, const int *ja, const double *a, TriEll &e) {
+  e.w = 0;
+  if (p.n <= 0 || p.max_row > 8) return;
+  const int w = p.max_row <= 4 ? 4 : 8;
+  e.ja = dev_alloc(sizeof(int) * (size_t)p.n * w);
+  e.a = dev_alloc(sizeof(double) * (size_t)p.n * w);
+  if (!e.ja || !e.a || !e.ja->p || !e.a->p) return;
+  const unsigned g = grid_for((size_t)p.n, 1, kBlock, 65536);
+  if (w == 4)
+    k_ell_pack<4><<<g, kBlock, 0, stream()>>>(p.n, p.desc->as<int4>(), ja, a, e.ja->as<int>(),
+                                              e.a->as<double>());
+  else
+    k_ell_pack<8><<<g, kBlock, 0, stream()>>>(p.n, p.desc->as<int4>(), ja, a, e.ja->as<int>(),
+                                              e.a->as<double>());
+  e.w = w;
+}
+
 void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
-                      const double *diag, const double *x, double *y, const int *done) {
+                      const double *diag, const double *x, double *y, const int *done,
+                      const TriEll *ell) {
   const int *order = p.order->as<int>();
+  if (ell && ell->w && options().sptrsv_mode == 0) {
+    for (int l = 0; l < p.nlevels; ++l) {
+      const int l0 = p.level_ptr[l], cnt = p.level_ptr[l + 1] - l0;
+      const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock);
+      const size_t off = (size_t)l0 * ell->w;
+      if (ell->w == 4)
+        k_tri_level_ell<4><<<g, kBlock, 0, stream()>>>(cnt, order + l0, ell->ja->as<int>() + off,
+                                                       ell->a->as<double>() + off, diag, x, y,
+                                                       done);
+      else
+        k_tri_level_ell<8><<<g, kBlock, 0, stream()>>>(cnt, order + l0, ell->ja->as<int>() + off,
+                                                       ell->a->as<double>() + off, diag, x, y,
+                                                       done);
+    }
+    return;
+  }
   // in place (x == y) needs x intact while y is filled: level launches then
   if (options().sptrsv_mode == 2 && x != y && p.nlevels > 0) {
     int *st = sf_state();

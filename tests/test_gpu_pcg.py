@@ -286,3 +286,43 @@ def test_sync_free_trisolve_pcg_is_bitwise_the_level_scheduled_one(kind, pname, 
             assert torch.equal(r1.x, r0.x)
     finally:
         S.set_option("sptrsv_mode", prev)
+
+
+def nine_point(nx, ny, shift=1e-8):
+    """2D 9-point graph Laplacian: L+D rows of up to 5 entries (the 8-wide
+    packed rows of the triangular solves)."""
+    from lap_amd.problems import from_coo
+    idx = np.arange(nx * ny).reshape(ny, nx)
+    rows, cols = [], []
+    for dy, dx in ((0, 1), (1, -1), (1, 0), (1, 1)):
+        a = idx[max(0, -dy):ny - max(0, dy), max(0, -dx):nx - max(0, dx)]
+        b = idx[max(0, dy):ny - max(0, -dy) if dy <= 0 else ny, max(0, dx):nx - max(0, -dx)]
+        a, b = a.ravel(), b.ravel()
+        rows.append(np.maximum(a, b))
+        cols.append(np.minimum(a, b))
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    deg = np.bincount(r, minlength=nx * ny) + np.bincount(c, minlength=nx * ny)
+    n = nx * ny
+    return from_coo(n, np.concatenate([r, np.arange(n)]), np.concatenate([c, np.arange(n)]),
+                    np.concatenate([-np.ones(r.shape[0]), deg + shift]), symmetric=True)
+
+
+@pytest.mark.parametrize("pname", ["grid3d_16", "nine_point"])
+@pytest.mark.parametrize("kind", ["ichol", "GS_std"])
+def test_packed_level_rows_are_bitwise_the_csr_level_kernel(kind, pname):
+    """The level launches read the triangles through rows packed in level
+    order (4 or 8 wide); the level-barrier mode still reads the CSR arrays.
+    Both give the same bits, so the whole PCG agrees."""
+    Ah = PROBLEMS[pname]() if pname in PROBLEMS else nine_point(70, 60)
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, kind)
+    b = S.to_dev(rhs_normal(Ah.n, 79))
+    try:
+        S.set_option("sptrsv_mode", 0)
+        r0 = S.pcg(A, b, P, tol=1e-10, maxit=5000)
+        S.set_option("sptrsv_mode", 2)
+        r2 = S.pcg(A, b, P, tol=1e-10, maxit=5000)
+    finally:
+        S.set_option("sptrsv_mode", 0)
+    assert r0.flag == 0 and (r0.iterations, r0.flag) == (r2.iterations, r2.flag)
+    assert np.array_equal(r0.history, r2.history) and torch.equal(r0.x, r2.x)
