@@ -208,6 +208,7 @@ struct Options {
   int sf_wg_per_cu = 8;    // sync-free solve: workgroups per CU
   int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
   int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
+  int mixed_vectors = 0;   // lap_pcg_mixed inner vectors: 0 FP32, 1 FP64
   int spmv_format = 0;     // PCG SpMV: 0 CSR values, 1 value-indexed when the
                            // matrix has at most 255 distinct values
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
@@ -283,10 +284,17 @@ struct TriPlan {
   int max_row = 0;             // longest row (entries, pivot included)
   DevBufPtr order;             // int[n]: rows sorted by level
   int max_level_rows = 0;
+  // Within each level, rows of at most kTriLongRow entries come first, then
+  // the longer ones: level l's long rows are order[level_long[l] ..
+  // level_ptr[l + 1]).  A long row is solved by a whole workgroup.
+  std::vector<int> level_long;  // host, nlevels
   bool has_diag = true;        // every row holds its pivot
   int nchunks = 0;             // sync-free mode: <= 64 rows of one level each
   DevBufPtr chunk_ptr;         // int[nchunks + 1], offsets into order
 };
+// Rows longer than this many entries (pivot included) are solved by a
+// workgroup each in the level launches.
+constexpr int kTriLongRow = 32;
 const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower);
 // A triangle's rows in level order as fixed-width records (w = 4 or 8
 // entries, padded with column -1): ja[t*w + k], a[t*w + k] for the row at
@@ -304,6 +312,13 @@ void tri_ell_build(const TriPlan &p, const int *ja, const double *a, TriEll &e);
 void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
                       const double *diag, const double *x, double *y, const int *ctl_done,
                       const TriEll *ell = nullptr);
+// The same level-scheduled solve over FP32 values and vectors with FP64 row
+// sums: y = (float)((x - sum a*y) / pivot), pivot in the row.
+void tri_solve32_launch(const TriPlan &p, const int *ja, const float *a, const float *x, float *y,
+                        const int *done);
+// FP32 values, FP64 vectors, FP64 row sums.
+void tri_solve32v64_launch(const TriPlan &p, const int *ja, const float *a, const double *x,
+                           double *y, const int *done);
 // Sync-free mode: nonzero once a dependency wait ran out (the result is then
 // invalid); reset by tri_clear_error.
 int tri_error();

@@ -19,6 +19,18 @@
 // matrix singular), every row sum and every dot product.  FP32 holds: the
 // off-diagonal values of A (own CSR without the diagonal), all inner
 // vectors, 1/diag for Jacobi and the IC0 factor.
+//
+// Row-sum-preserving rounding: the inner operator's FP64 diagonal is A's
+// diagonal plus, per row, the sum of (a_ij - fl32(a_ij)) over the row's
+// off-diagonals, so every row of the inner operator sums to the same value
+// as A's row (to FP64 rounding).  The rounding error is then
+// E = sum_{i<j} e_ij (u_i - u_j)(u_i - u_j)^T with |e_ij| <= 2^-24 |a_ij|,
+// and for a graph Laplacian plus a non-negative diagonal shift
+// |x^T E x| <= 2^-24 x^T A x: the inner operator stays SPD and within a
+// factor 1 +- 6e-8 of A however small the shift.  Plain rounding moves a
+// row sum by up to ~2^-24 sqrt(deg) |a_i|, which for a power-law graph's
+// hub rows (degree ~1e4) exceeds a 1e-6 relative diagonal shift and makes
+// the FP32 operator indefinite.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -61,28 +73,30 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(int n, const double *__restri
     d[i] = (float)s[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_zero_f32(int n, float *__restrict__ d) {
+template <class VT>
+__global__ __launch_bounds__(kBlock) void k_zero_v(int n, VT *__restrict__ d) {
   const size_t stride = (size_t)gridDim.x * kBlock;
   for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n; i += stride)
-    d[i] = 0.0f;
+    d[i] = VT(0);
 }
 
-// FP64 residual epilogue: r32 = (float)(b - A x); sums of r.r (FP64) and,
-// with Jacobi, r.w and p = w (first direction) for w = d_r r.
+// FP64 residual epilogue: r = (VT)(b - A x); sums of r.r (FP64) and, with
+// Jacobi, r.w and p = w (first direction) for w = d_r r.
+template <class VT>
 struct Resid64Epi {
   const double *rhs;
-  float *r32;
-  const float *d_r;  // null: no Jacobi terms
-  float *p_init;     // null: do not set p
+  VT *r32;
+  const VT *d_r;  // null: no Jacobi terms
+  VT *p_init;     // null: do not set p
   double rr, rz;
   __device__ double init(int) const { return 0.0; }
   __device__ void store(int row, double s) {
     const double v = -(s - rhs[row]);
-    const float rf = (float)v;
+    const VT rf = (VT)v;
     r32[row] = rf;
     rr += v * v;
     if (d_r) {
-      const float wf = d_r[row] * rf;
+      const VT wf = d_r[row] * rf;
       rz += (double)rf * (double)wf;
       if (p_init) p_init[row] = wf;
     }
@@ -90,18 +104,18 @@ struct Resid64Epi {
 };
 
 // r = b - A x (FP64), initial version: always runs.
-template <int BS, int TILE>
+template <int BS, int TILE, class VT>
 __global__ __launch_bounds__(BS) void k_mx_init(int nblk, const int *__restrict__ rowblk,
                                                 const int *__restrict__ ia,
                                                 const int *__restrict__ ja,
                                                 const double *__restrict__ a,
                                                 const double *__restrict__ x,
-                                                const double *__restrict__ rhs, float *r32,
-                                                const float *d_r, float *p32, double *part,
+                                                const double *__restrict__ rhs, VT *r32,
+                                                const VT *d_r, VT *p32, double *part,
                                                 int walk) {
   __shared__ double prod[TILE];
   __shared__ double red[BS / kWave];
-  Resid64Epi epi{rhs, r32, d_r, p32, 0.0, 0.0};
+  Resid64Epi<VT> epi{rhs, r32, d_r, p32, 0.0, 0.0};
   spmv_walk<BS, TILE, 0>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
   const double t_rz = block_sum<BS>(epi.rz, red);
   const double t_rr = block_sum<BS>(epi.rr, red);
@@ -136,29 +150,30 @@ __global__ __launch_bounds__(BS) void k_mx_start(const double *part, int cnt, do
   }
 }
 
-struct Pq32Epi {
-  float *q;
-  const float *p;
+template <class VT>
+struct PqMxEpi {
+  VT *q;
+  const VT *p;
   const double *diag;
   double acc;
   __device__ double init(int row) const { return diag[row] * (double)p[row]; }
   __device__ void store(int row, double s) {
-    const float qf = (float)s;
+    const VT qf = (VT)s;
     q[row] = qf;
     acc += (double)p[row] * (double)qf;
   }
 };
 
-// q32 = A32 p32 (FP32 off-diagonals + FP64 diagonal, FP64 row sums);
-// partials of p.q.  Workgroup 0 commits the pending control fields.
-template <int BS, int TILE>
+// q = A32 p (FP32 off-diagonals + FP64 diagonal, FP64 row sums); partials
+// of p.q.  Workgroup 0 commits the pending control fields.
+template <int BS, int TILE, class VT>
 __global__ __launch_bounds__(BS) void k_mx_spmv_pq(int nblk, const int *__restrict__ rowblk,
                                                    const int *__restrict__ ia,
                                                    const int *__restrict__ ja,
                                                    const float *__restrict__ a,
                                                    const double *__restrict__ diag,
-                                                   const float *__restrict__ p,
-                                                   float *__restrict__ q, MixCtl *ctl,
+                                                   const VT *__restrict__ p,
+                                                   VT *__restrict__ q, MixCtl *ctl,
                                                    double *part, int walk) {
   if (ctl->done) return;
   if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->pend_valid) {
@@ -170,12 +185,12 @@ __global__ __launch_bounds__(BS) void k_mx_spmv_pq(int nblk, const int *__restri
   }
   __shared__ double prod[TILE];
   __shared__ double red[BS / kWave];
-  Pq32Epi epi{q, p, diag, 0.0};
+  PqMxEpi<VT> epi{q, p, diag, 0.0};
   const BlockWalk w = block_walk(nblk, walk);
   for (int s = w.begin; s < w.end; s += w.step) {
     const int blk = w.block(s);
     if (blk >= nblk) continue;
-    spmv_block_rows<BS, TILE, Pq32Epi, float, float>(blk, rowblk, ia, ja, a, p, 1.0, prod, red,
+    spmv_block_rows<BS, TILE, PqMxEpi<VT>, float, VT>(blk, rowblk, ia, ja, a, p, 1.0, prod, red,
                                                       epi);
   }
   const double t = block_sum<BS>(epi.acc, red);
@@ -184,15 +199,15 @@ __global__ __launch_bounds__(BS) void k_mx_spmv_pq(int nblk, const int *__restri
 
 // The same over the value-indexed copy: off-diagonal value = dict[vi[e]],
 // the table staged in LDS; bitwise the same products and sums.
-template <int BS, int TILE>
+template <int BS, int TILE, class VT>
 __global__ __launch_bounds__(BS) void k_mx_spmv_pq_vi(int nblk, const int *__restrict__ rowblk,
                                                       const int *__restrict__ ia,
                                                       const int *__restrict__ ja,
                                                       const unsigned char *__restrict__ vi,
                                                       const float *__restrict__ dict, int nd,
                                                       const double *__restrict__ diag,
-                                                      const float *__restrict__ p,
-                                                      float *__restrict__ q, MixCtl *ctl,
+                                                      const VT *__restrict__ p,
+                                                      VT *__restrict__ q, MixCtl *ctl,
                                                       double *part, int walk) {
   if (ctl->done) return;
   if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->pend_valid) {
@@ -207,12 +222,12 @@ __global__ __launch_bounds__(BS) void k_mx_spmv_pq_vi(int nblk, const int *__res
   __shared__ float table[256];
   for (int i = threadIdx.x; i < nd; i += BS) table[i] = dict[i];
   __syncthreads();
-  Pq32Epi epi{q, p, diag, 0.0};
+  PqMxEpi<VT> epi{q, p, diag, 0.0};
   const BlockWalk w = block_walk(nblk, walk);
   for (int s = w.begin; s < w.end; s += w.step) {
     const int blk = w.block(s);
     if (blk >= nblk) continue;
-    spmv_block_rows_vi<BS, TILE, Pq32Epi, float, float>(blk, rowblk, ia, ja, vi, table, p, 1.0,
+    spmv_block_rows_vi<BS, TILE, PqMxEpi<VT>, float, VT>(blk, rowblk, ia, ja, vi, table, p, 1.0,
                                                         prod, red, epi);
   }
   const double t = block_sum<BS>(epi.acc, red);
@@ -220,14 +235,14 @@ __global__ __launch_bounds__(BS) void k_mx_spmv_pq_vi(int nblk, const int *__res
 }
 
 // alpha = rho/(p.q); x32 += alpha p; r32 -= alpha q (each rounded once from
-// FP64); partials of r.r and, with Jacobi, r.w for w = d_r r (stride 2:
-// [rz, rr]).
-template <int BS, bool JAC>
-__global__ __launch_bounds__(BS) void k_mx_update(int n, float *__restrict__ x32,
-                                                  float *__restrict__ r32,
-                                                  const float *__restrict__ p,
-                                                  const float *__restrict__ q,
-                                                  const float *__restrict__ d_r,
+// FP64 to VT); partials of r.r and, with Jacobi, r.w for w = d_r r (stride
+// 2: [rz, rr]).
+template <int BS, bool JAC, class VT>
+__global__ __launch_bounds__(BS) void k_mx_update(int n, VT *__restrict__ x32,
+                                                  VT *__restrict__ r32,
+                                                  const VT *__restrict__ p,
+                                                  const VT *__restrict__ q,
+                                                  const VT *__restrict__ d_r,
                                                   const MixCtl *ctl, const double *pq_part,
                                                   int pq_cnt, double *part) {
   if (ctl->done) return;
@@ -237,12 +252,12 @@ __global__ __launch_bounds__(BS) void k_mx_update(int n, float *__restrict__ x32
   double rz = 0.0, rr = 0.0;
   const size_t stride = (size_t)gridDim.x * BS;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
-    x32[i] = (float)((double)x32[i] + alpha * (double)p[i]);
-    const float rf = (float)((double)r32[i] - alpha * (double)q[i]);
+    x32[i] = (VT)((double)x32[i] + alpha * (double)p[i]);
+    const VT rf = (VT)((double)r32[i] - alpha * (double)q[i]);
     r32[i] = rf;
     rr += (double)rf * (double)rf;
     if (JAC) {
-      const float wf = d_r[i] * rf;
+      const VT wf = d_r[i] * rf;
       rz += (double)rf * (double)wf;
     }
   }
@@ -255,9 +270,9 @@ __global__ __launch_bounds__(BS) void k_mx_update(int n, float *__restrict__ x32
 }
 
 // refinement, step 1: x += x32, x32 = 0 (only when this iteration refines)
-template <int BS>
+template <int BS, class VT>
 __global__ __launch_bounds__(BS) void k_mx_fold(int n, double *__restrict__ x,
-                                                float *__restrict__ x32, const MixCtl *ctl,
+                                                VT *__restrict__ x32, const MixCtl *ctl,
                                                 const double *upd_part, int upd_cnt) {
   if (ctl->done) return;
   __shared__ double red[BS / kWave];
@@ -266,19 +281,19 @@ __global__ __launch_bounds__(BS) void k_mx_fold(int n, double *__restrict__ x,
   const size_t stride = (size_t)gridDim.x * BS;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
     x[i] += (double)x32[i];
-    x32[i] = 0.0f;
+    x32[i] = VT(0);
   }
 }
 
 // refinement, step 2: r32 = (float)(b - A x) in FP64; partials [rz, rr]
-template <int BS, int TILE>
+template <int BS, int TILE, class VT>
 __global__ __launch_bounds__(BS) void k_mx_resid(int nblk, const int *__restrict__ rowblk,
                                                  const int *__restrict__ ia,
                                                  const int *__restrict__ ja,
                                                  const double *__restrict__ a,
                                                  const double *__restrict__ x,
-                                                 const double *__restrict__ rhs, float *r32,
-                                                 const float *d_r, const MixCtl *ctl,
+                                                 const double *__restrict__ rhs, VT *r32,
+                                                 const VT *d_r, const MixCtl *ctl,
                                                  const double *upd_part, int upd_cnt,
                                                  double *part, int walk) {
   if (ctl->done) return;
@@ -286,7 +301,7 @@ __global__ __launch_bounds__(BS) void k_mx_resid(int nblk, const int *__restrict
   __shared__ double red[BS / kWave];
   const double rr_rec = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
   if (!need_refine(rr_rec, ctl->iter + 1, ctl)) return;
-  Resid64Epi epi{rhs, r32, d_r, nullptr, 0.0, 0.0};
+  Resid64Epi<VT> epi{rhs, r32, d_r, nullptr, 0.0, 0.0};
   spmv_walk<BS, TILE, 0>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
   const double t_rz = block_sum<BS>(epi.rz, red);
   const double t_rr = block_sum<BS>(epi.rr, red);
@@ -328,10 +343,10 @@ __device__ __forceinline__ MxStep mx_step(const MixCtl *ctl, const double *upd_p
 }
 
 // Jacobi: convergence bookkeeping, beta, p = beta p + d_r r
-template <int BS>
-__global__ __launch_bounds__(BS) void k_mx_jac_p(int n, const float *__restrict__ r32,
-                                                 const float *__restrict__ d_r,
-                                                 float *__restrict__ p, MixCtl *ctl,
+template <int BS, class VT>
+__global__ __launch_bounds__(BS) void k_mx_jac_p(int n, const VT *__restrict__ r32,
+                                                 const VT *__restrict__ d_r,
+                                                 VT *__restrict__ p, MixCtl *ctl,
                                                  double *hist, const double *upd_part,
                                                  int upd_cnt, const double *ref_part,
                                                  int ref_cnt) {
@@ -358,15 +373,15 @@ __global__ __launch_bounds__(BS) void k_mx_jac_p(int n, const float *__restrict_
   const double beta = s.rz / ctl->rho;
   const size_t stride = (size_t)gridDim.x * BS;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
-    const float wf = d_r[i] * r32[i];
-    p[i] = (float)((double)p[i] * beta + (double)wf);
+    const VT wf = d_r[i] * r32[i];
+    p[i] = (VT)((double)p[i] * beta + (double)wf);
   }
 }
 
 // generic: partials of r.w
-template <int BS>
-__global__ __launch_bounds__(BS) void k_mx_rw(int n, const float *__restrict__ r,
-                                              const float *__restrict__ w, const MixCtl *ctl,
+template <int BS, class VT>
+__global__ __launch_bounds__(BS) void k_mx_rw(int n, const VT *__restrict__ r,
+                                              const VT *__restrict__ w, const MixCtl *ctl,
                                               double *part) {
   if (ctl->done) return;
   __shared__ double red[BS / kWave];
@@ -379,9 +394,9 @@ __global__ __launch_bounds__(BS) void k_mx_rw(int n, const float *__restrict__ r
 }
 
 // generic: rho, p = w (first) or beta p + w
-template <int BS>
-__global__ __launch_bounds__(BS) void k_mx_pupd(int n, const float *__restrict__ w,
-                                                float *__restrict__ p, MixCtl *ctl,
+template <int BS, class VT>
+__global__ __launch_bounds__(BS) void k_mx_pupd(int n, const VT *__restrict__ w,
+                                                VT *__restrict__ p, MixCtl *ctl,
                                                 const double *part, int cnt) {
   if (ctl->done) return;
   __shared__ double red[BS / kWave];
@@ -391,7 +406,7 @@ __global__ __launch_bounds__(BS) void k_mx_pupd(int n, const float *__restrict__
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rho = rho;
   const size_t stride = (size_t)gridDim.x * BS;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride)
-    p[i] = first ? w[i] : (float)((double)p[i] * beta + (double)w[i]);
+    p[i] = first ? w[i] : (VT)((double)p[i] * beta + (double)w[i]);
 }
 
 // generic, one workgroup: convergence bookkeeping
@@ -419,33 +434,9 @@ __global__ __launch_bounds__(BS) void k_mx_conv(MixCtl *ctl, double *hist,
   }
 }
 
-// FP32 triangular solve level (FP64 row sums), pivot = in-row diagonal
-__global__ __launch_bounds__(kBlock) void k_tri32_level(int cnt, const int *__restrict__ rows,
-                                                        const int *__restrict__ ia,
-                                                        const int *__restrict__ ja,
-                                                        const float *__restrict__ a,
-                                                        const float *x, float *y,
-                                                        const int *__restrict__ done) {
-  if (done && *done) return;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= cnt) return;
-  const int row = rows[t];
-  double acc = (double)x[row];
-  double piv = 0.0;
-  const int e1 = ia[row + 1];
-  for (int e = ia[row]; e < e1; ++e) {
-    const int c = ja[e];
-    if (c == row) {
-      piv = (double)a[e];
-      continue;
-    }
-    acc -= (double)a[e] * (double)y[c];
-  }
-  y[row] = (float)(acc / piv);
-}
-
-__global__ __launch_bounds__(kBlock) void k_copy_f32(int n, const float *__restrict__ s,
-                                                     float *__restrict__ d,
+template <class VT>
+__global__ __launch_bounds__(kBlock) void k_copy_v(int n, const VT *__restrict__ s,
+                                                   VT *__restrict__ d,
                                                      const int *__restrict__ done) {
   if (done && *done) return;
   const size_t stride = (size_t)gridDim.x * kBlock;
@@ -483,14 +474,18 @@ MixPlan *mix_plan(int n, int nnz, const int *ia, const int *ja, const double *a)
   jo.reserve((size_t)total);
   ao.reserve((size_t)total);
   for (int i = 0; i < n; ++i) {
+    double comp = 0.0;  // what rounding took from the row's off-diagonals
     for (int e = h_ia[i]; e < h_ia[i + 1]; ++e) {
       if (h_ja[e] == i) {
         dg[i] += h_a[e];
       } else {
+        const float v = (float)h_a[e];
         jo.push_back(h_ja[e]);
-        ao.push_back((float)h_a[e]);
+        ao.push_back(v);
+        comp += h_a[e] - (double)v;
       }
     }
+    dg[i] += comp;
     io[i + 1] = (int)jo.size();
   }
   auto P = std::make_unique<MixPlan>();
@@ -553,6 +548,29 @@ struct MixOut {
   double res0 = 0, res = 0, gpu_ms = 0;
 };
 
+// 1/diag for the inner Jacobi step in the inner vectors' type
+template <class VT>
+const VT *inner_d_r(MixPlan *M, const double *d_r, int n);
+template <>
+const float *inner_d_r<float>(MixPlan *M, const double *d_r, int n) {
+  return to_f32_cached(M->dr32, d_r, n);
+}
+template <>
+const double *inner_d_r<double>(MixPlan *, const double *d_r, int) {
+  return d_r;
+}
+
+// FP32 triangular solve of the inner IC0 step, vectors in VT
+inline void tri_inner(const TriPlan &T, const int *ja, const float *a, const float *x, float *y,
+                      const int *done) {
+  tri_solve32_launch(T, ja, a, x, y, done);
+}
+inline void tri_inner(const TriPlan &T, const int *ja, const float *a, const double *x, double *y,
+                      const int *done) {
+  tri_solve32v64_launch(T, ja, a, x, y, done);
+}
+
+template <class VT>
 static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double *a, double *x,
                      const double *b, double tol, lap_pdata *pd, int maxit, double delta,
                      double *h_hist, MixOut &out) {
@@ -579,13 +597,13 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   if (!A64) return lap_last_status();
   MixPlan *M = mix_plan(n, nnz, ia, ja, a);
   if (!M) return lap_last_status();
-  const float *d_r32 = nullptr;
+  const VT *d_r32 = nullptr;
   if (jac) {
     if (!is_device_ptr(pd->d_r)) {
       set_error(LAP_ERR_ARG, "pcg_mixed: prec_data->d_r is not device memory");
       return LAP_ERR_ARG;
     }
-    d_r32 = to_f32_cached(M->dr32, pd->d_r, n);
+    d_r32 = inner_d_r<VT>(M, pd->d_r, n);
   }
   const IcPlan *icp = nullptr;
   float *l32 = nullptr, *u32 = nullptr;
@@ -610,9 +628,9 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   }
 
   auto fvec = [&](const char *tag) {
-    return reinterpret_cast<float *>(scratch_vec(tag, (size_t)n / 2 + 1));
+    return reinterpret_cast<VT *>(scratch_vec(tag, ((size_t)n * sizeof(VT) + 7) / 8 + 1));
   };
-  float *r32 = fvec("mx_r"), *p32 = fvec("mx_p"), *q32 = fvec("mx_q"), *x32 = fvec("mx_x"),
+  VT *r32 = fvec("mx_r"), *p32 = fvec("mx_p"), *q32 = fvec("mx_q"), *x32 = fvec("mx_x"),
         *w32 = fvec("mx_w"), *aux32 = fvec("mx_aux");
   MixCtl *ctl = reinterpret_cast<MixCtl *>(scratch_vec("mx_ctl", sizeof(MixCtl) / 8 + 1));
   double *hist = scratch_vec("mx_hist", (size_t)maxit + 2);
@@ -631,8 +649,8 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   LAP_HIP(hipEventCreate(&ev0));
   LAP_HIP(hipEventCreate(&ev1));
   LAP_HIP(hipEventRecord(ev0, stream()));
-  k_zero_f32<<<gV, kBlock, 0, stream()>>>(n, x32);
-  k_mx_init<kBlock, kSpmvTile><<<gA, kBlock, 0, stream()>>>(
+  k_zero_v<VT><<<gV, kBlock, 0, stream()>>>(n, x32);
+  k_mx_init<kBlock, kSpmvTile, VT><<<gA, kBlock, 0, stream()>>>(
       A64->nblocks, A64->rowblk->as<int>(), ia, ja, a, x, b, r32, d_r32, jac ? p32 : nullptr,
       Q0, walk);
   k_mx_start<kBlock><<<1, kBlock, 0, stream()>>>(Q0, (int)gA, tol, delta, ctl, hist);
@@ -641,51 +659,42 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   const int *io = M->ia_o->as<int>(), *jo = M->ja_o->as<int>();
   const float *ao = M->a32_o->as<float>();
   const double *dg = M->diag->as<double>();
-  auto tri32 = [&](const TriPlan &T, const int *tia, const int *tja, const float *ta,
-                   const float *in, float *outv) {
-    const int *order = T.order->as<int>();
-    for (int l = 0; l < T.nlevels; ++l) {
-      const int cnt = T.level_ptr[l + 1] - T.level_ptr[l];
-      k_tri32_level<<<(cnt + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(
-          cnt, order + T.level_ptr[l], tia, tja, ta, in, outv, done);
-    }
-  };
   // value-indexed inner SpMV when spmv_format asks for it and the table fits
   const bool vi32 = options().spmv_format == 1 && M->nd > 0;
   auto spmv32 = [&]() {
     if (vi32)
-      k_mx_spmv_pq_vi<kBlock, kSpmvTile><<<gO, kBlock, 0, stream()>>>(
+      k_mx_spmv_pq_vi<kBlock, kSpmvTile, VT><<<gO, kBlock, 0, stream()>>>(
           M->off->nblocks, rb_o, io, jo, M->vi_o->as<unsigned char>(), M->dict_o->as<float>(),
           M->nd, dg, p32, q32, ctl, Q1, walk);
     else
-      k_mx_spmv_pq<kBlock, kSpmvTile><<<gO, kBlock, 0, stream()>>>(M->off->nblocks, rb_o, io,
+      k_mx_spmv_pq<kBlock, kSpmvTile, VT><<<gO, kBlock, 0, stream()>>>(M->off->nblocks, rb_o, io,
                                                                    jo, ao, dg, p32, q32, ctl,
                                                                    Q1, walk);
   };
   auto iteration = [&]() {
     if (jac) {
       spmv32();
-      k_mx_update<kBlock, true><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, d_r32, ctl,
-                                                             Q1, (int)gO, Q2);
-      k_mx_fold<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
-      k_mx_resid<kBlock, kSpmvTile><<<gA, kBlock, 0, stream()>>>(
+      k_mx_update<kBlock, true, VT><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, d_r32,
+                                                                 ctl, Q1, (int)gO, Q2);
+      k_mx_fold<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
+      k_mx_resid<kBlock, kSpmvTile, VT><<<gA, kBlock, 0, stream()>>>(
           A64->nblocks, rb_a, ia, ja, a, x, b, r32, d_r32, ctl, Q2, (int)gV, Q3, walk);
-      k_mx_jac_p<kBlock><<<gV, kBlock, 0, stream()>>>(n, r32, d_r32, p32, ctl, hist, Q2,
-                                                      (int)gV, Q3, (int)gA);
+      k_mx_jac_p<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, r32, d_r32, p32, ctl, hist, Q2,
+                                                          (int)gV, Q3, (int)gA);
     } else {
       if (ic) {
-        tri32(*icp->lplan, icp->lia->as<int>(), icp->lja->as<int>(), l32, r32, aux32);
-        tri32(*icp->uplan, icp->uia->as<int>(), icp->uja->as<int>(), u32, aux32, w32);
+        tri_inner(*icp->lplan, icp->lja->as<int>(), l32, r32, aux32, done);
+        tri_inner(*icp->uplan, icp->uja->as<int>(), u32, aux32, w32, done);
       } else {
-        k_copy_f32<<<gV, kBlock, 0, stream()>>>(n, r32, w32, done);
+        k_copy_v<VT><<<gV, kBlock, 0, stream()>>>(n, r32, w32, done);
       }
-      k_mx_rw<kBlock><<<gV, kBlock, 0, stream()>>>(n, r32, w32, ctl, Q0);
-      k_mx_pupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, w32, p32, ctl, Q0, (int)gV);
+      k_mx_rw<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, r32, w32, ctl, Q0);
+      k_mx_pupd<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, w32, p32, ctl, Q0, (int)gV);
       spmv32();
-      k_mx_update<kBlock, false><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, nullptr,
-                                                              ctl, Q1, (int)gO, Q2);
-      k_mx_fold<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
-      k_mx_resid<kBlock, kSpmvTile><<<gA, kBlock, 0, stream()>>>(
+      k_mx_update<kBlock, false, VT><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32,
+                                                                  nullptr, ctl, Q1, (int)gO, Q2);
+      k_mx_fold<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
+      k_mx_resid<kBlock, kSpmvTile, VT><<<gA, kBlock, 0, stream()>>>(
           A64->nblocks, rb_a, ia, ja, a, x, b, r32, nullptr, ctl, Q2, (int)gV, Q3, walk);
       k_mx_conv<kBlock><<<1, kBlock, 0, stream()>>>(ctl, hist, Q2, (int)gV, Q3, (int)gA);
     }
@@ -773,7 +782,10 @@ extern "C" int lap_pcg_mixed(int n, int nnz, const int *ia, const int *ja, const
                              int maxit, double delta, double *history, lap_solve_info *info) {
   MixOut out;
   if (delta == 0.0) delta = 0.01;
-  const int st = run_mixed(n, nnz, ia, ja, a, x, b, tol, prec_data, maxit, delta, history, out);
+  const int st =
+      options().mixed_vectors == 1
+          ? run_mixed<double>(n, nnz, ia, ja, a, x, b, tol, prec_data, maxit, delta, history, out)
+          : run_mixed<float>(n, nnz, ia, ja, a, x, b, tol, prec_data, maxit, delta, history, out);
   if (info) {
     info->iterations = out.iterations;
     info->flag = st == LAP_OK ? out.flag : 2;

@@ -3,9 +3,10 @@
 //
 // Level scheduling: the analysis assigns each row the length of its longest
 // dependency chain (level) and sorts rows by level; a solve runs the levels
-// in order, one launch per level, one lane per row.  Each lane subtracts its
-// row's off-diagonal terms in CSR order and divides by the pivot, so every
-// row is computed with exactly the reference's sequential arithmetic
+// in order, one launch per level, one lane per row (a workgroup per row for
+// rows of more than kTriLongRow entries).  Each row's off-diagonal terms are
+// subtracted in CSR order and the sum divided by the pivot, so every row is
+// computed with exactly the reference's sequential arithmetic
 // (simple_blas.c:46-85) and the result equals the sequential solve bitwise.
 //
 // Sync-free mode (option sptrsv_mode 1): one launch per solve.  Rows are cut
@@ -46,31 +47,84 @@ namespace {
 
 // desc[t] = {row, first nonzero, end}: one load gives the row and its range
 // (the row pointers are not read in the solve).
+//
+// One launch per level.  The level's short rows (the first `cnt` records)
+// take one lane each; each of its `nlong` long rows (the records after them)
+// takes a workgroup of its own: the workgroup forms the row's products a
+// chunk at a time into LDS and lane 0 subtracts them in CSR order (the
+// pivot's slot holds +0.0, and x - (+0.0) == x bit for bit), so a long row
+// gets exactly the lane path's arithmetic while its loads proceed in
+// parallel.  AT/VT (value/vector type) = double/double: the FP64 solve;
+// float/float or float/double: FP32 values with FP64 products and sums (the
+// mixed-precision inner solve).
+constexpr int kTriChunk = 2048;
+
+template <class AT, class VT>
 __global__ __launch_bounds__(kBlock) void k_tri_level(
-    int cnt, const int4 *__restrict__ desc, const int *__restrict__ ja,
-    const double *__restrict__ a, const double *__restrict__ diag, const double *x, double *y,
+    int cnt, int nlong, const int4 *__restrict__ desc, const int *__restrict__ ja,
+    const AT *__restrict__ a, const double *__restrict__ diag, const VT *x, VT *y,
     const int *__restrict__ done) {
   if (done && *done) return;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= cnt) return;
-  const int4 d = desc[t];
-  const int row = d.x;
-  double acc = x[row];
-  double piv = 0.0;
-  bool in_row = false;
-  const int e1 = d.z;
-  for (int e = d.y; e < e1; ++e) {
-    const int c = ja[e];
-    if (c == row) {
-      piv = a[e];
-      in_row = true;
-      continue;
+  const int gshort = (cnt + kBlock - 1) / kBlock;
+  if ((int)blockIdx.x < gshort) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= cnt) return;
+    const int4 d = desc[t];
+    const int row = d.x;
+    double acc = (double)x[row];
+    double piv = 0.0;
+    bool in_row = false;
+    const int e1 = d.z;
+    for (int e = d.y; e < e1; ++e) {
+      const int c = ja[e];
+      if (c == row) {
+        piv = (double)a[e];
+        in_row = true;
+        continue;
+      }
+      acc -= (double)a[e] * (double)y[c];
     }
-    acc -= a[e] * y[c];
+    // diag is only passed (and only read) for matrices without in-row pivots
+    if (!in_row && diag) piv = diag[row];
+    y[row] = (VT)(acc / piv);
+    return;
   }
-  // diag is only passed (and only read) for matrices without in-row pivots
-  if (!in_row && diag) piv = diag[row];
-  y[row] = acc / piv;
+  const int lr = (int)blockIdx.x - gshort;
+  if (lr >= nlong) return;
+  __shared__ double prod[kTriChunk];
+  __shared__ int dpos;
+  const int4 d = desc[cnt + lr];
+  const int row = d.x;
+  if (threadIdx.x == 0) dpos = -1;
+  double acc = (double)x[row];
+  for (int base = d.y; base < d.z; base += kTriChunk) {
+    const int m = min(kTriChunk, d.z - base);
+    __syncthreads();  // the previous chunk has been consumed
+    for (int k = threadIdx.x; k < m; k += kBlock) {
+      const int e = base + k;
+      const int c = ja[e];
+      if (c == row) {
+        prod[k] = 0.0;
+        atomicMax(&dpos, e);  // the last pivot entry, as the lane path keeps
+      } else {
+        prod[k] = (double)a[e] * (double)y[c];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll 8
+      for (int k = 0; k < m; ++k) acc -= prod[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double piv = 0.0;
+    if (dpos >= 0)
+      piv = (double)a[dpos];
+    else if (diag)
+      piv = diag[row];
+    y[row] = (VT)(acc / piv);
+  }
 }
 
 // Packed rows: lane t loads its row's w columns and values together (one
@@ -348,8 +402,21 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
   p->level_ptr.assign((size_t)nlev + 1, 0);
   for (int i = 0; i < n; ++i) p->level_ptr[level[i] + 1]++;
   for (int l = 0; l < nlev; ++l) p->level_ptr[l + 1] += p->level_ptr[l];
-  std::vector<int> fill(p->level_ptr.begin(), p->level_ptr.end() - 1), order((size_t)n);
-  for (int i = 0; i < n; ++i) order[fill[level[i]]++] = i;
+  // within a level: the rows of at most kTriLongRow entries, then the rest,
+  // each in row order
+  std::vector<int> nshort((size_t)nlev, 0);
+  for (int i = 0; i < n; ++i)
+    if (ia[i + 1] - ia[i] <= kTriLongRow) nshort[level[i]]++;
+  p->level_long.resize((size_t)nlev);
+  std::vector<int> fill(p->level_ptr.begin(), p->level_ptr.end() - 1), fill_long((size_t)nlev),
+      order((size_t)n);
+  for (int l = 0; l < nlev; ++l) fill_long[l] = p->level_long[l] = p->level_ptr[l] + nshort[l];
+  for (int i = 0; i < n; ++i) {
+    if (ia[i + 1] - ia[i] <= kTriLongRow)
+      order[fill[level[i]]++] = i;
+    else
+      order[fill_long[level[i]]++] = i;
+  }
   for (int l = 0; l < nlev; ++l)
     p->max_level_rows = std::max(p->max_level_rows, p->level_ptr[l + 1] - p->level_ptr[l]);
   for (int i = 0; i < n; ++i) p->max_row = std::max(p->max_row, ia[i + 1] - ia[i]);
@@ -472,10 +539,33 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
   }
   const int4 *desc = p.desc->as<int4>();
   for (int l = 0; l < p.nlevels; ++l) {
-    const int cnt = p.level_ptr[l + 1] - p.level_ptr[l];
-    const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock);
-    k_tri_level<<<g, kBlock, 0, stream()>>>(cnt, desc + p.level_ptr[l], ja, a, diag, x, y, done);
+    const int cnt = p.level_long[l] - p.level_ptr[l], nlong = p.level_ptr[l + 1] - p.level_long[l];
+    const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock + nlong);
+    k_tri_level<double, double><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l],
+                                                            ja, a, diag, x, y, done);
   }
+}
+
+template <class VT>
+static void tri_mixed_launch(const TriPlan &p, const int *ja, const float *a, const VT *x, VT *y,
+                             const int *done) {
+  const int4 *desc = p.desc->as<int4>();
+  for (int l = 0; l < p.nlevels; ++l) {
+    const int cnt = p.level_long[l] - p.level_ptr[l], nlong = p.level_ptr[l + 1] - p.level_long[l];
+    const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock + nlong);
+    k_tri_level<float, VT><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l], ja, a,
+                                                       nullptr, x, y, done);
+  }
+}
+
+void tri_solve32_launch(const TriPlan &p, const int *ja, const float *a, const float *x, float *y,
+                        const int *done) {
+  tri_mixed_launch<float>(p, ja, a, x, y, done);
+}
+
+void tri_solve32v64_launch(const TriPlan &p, const int *ja, const float *a, const double *x,
+                           double *y, const int *done) {
+  tri_mixed_launch<double>(p, ja, a, x, y, done);
 }
 
 static void tri_entry(int n, int nnz, const int *ia, const int *ja, const double *a,

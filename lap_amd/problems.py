@@ -291,3 +291,68 @@ def read_rhs(path: str, n: int) -> np.ndarray:
         while line.startswith("%"):
             line = f.readline()
         return np.loadtxt(f, max_rows=n, usecols=0, ndmin=1)
+
+
+
+def chung_lu(n: int, gamma: float = 2.1, mean_degree: float = 16.0, cap: float = 20000.0,
+             seed: int = 12345, diag_factor: float = 1.0 + 1e-6) -> CSR:
+    """Weighted Chung-Lu power-law graph Laplacian, the mixed-precision
+    workload on a graph with very uneven rows.
+
+    Expected degrees w_i = c * i^(-1/(gamma-1)) (i = 1..n), c chosen for a
+    mean of `mean_degree`, then capped at `cap`; n*mean_degree/2 edges whose
+    endpoints are both drawn in proportion to w (the fast Chung-Lu sampler);
+    self-loops dropped, duplicate edges merged, the result symmetric.  Edge
+    weights are lognormal(0, 1) and the diagonal is diag_factor times the
+    row's weight sum (a 1e-6 relative shift makes the M-matrix strictly
+    diagonally dominant).  A vertex that received no edge gets diagonal 1.
+    Rows are sorted by column."""
+    rng = np.random.default_rng(seed)
+    w = np.arange(1, n + 1, dtype=np.float64) ** (-1.0 / (gamma - 1.0))
+    w *= mean_degree / w.mean()
+    np.minimum(w, cap, out=w)
+    m = int(n * mean_degree / 2)
+    # m independent endpoint pairs: each side's vertex counts are
+    # multinomial(m, w/sum w), and a random pairing of the two sides makes
+    # the pairs independent
+    prob = w / w.sum()
+    verts = np.arange(n, dtype=np.int64)
+    src = np.repeat(verts, rng.multinomial(m, prob))
+    dst = rng.permutation(np.repeat(verts, rng.multinomial(m, prob)))
+    keep = src != dst
+    lo = np.minimum(src[keep], dst[keep]).astype(np.int64)
+    hi = np.maximum(src[keep], dst[keep]).astype(np.int64)
+    del src, dst, keep
+    key = np.unique(lo * n + hi)  # sorted by (lo, hi): the upper triangle in CSR order
+    lo, hi = key // n, key % n
+    del key
+    ew = rng.lognormal(0.0, 1.0, lo.shape[0])
+    sums = np.bincount(lo, ew, n) + np.bincount(hi, ew, n)
+    cl = np.bincount(hi, minlength=n)  # entries left of the diagonal per row
+    cu = np.bincount(lo, minlength=n)  # entries right of it
+    ia = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(cl + cu + 1, out=ia[1:])
+    if ia[-1] >= 2**31:
+        raise ValueError("nnz exceeds int32 CSR")
+    nnz = int(ia[-1])
+    ja = np.empty(nnz, dtype=np.int32)
+    a = np.empty(nnz, dtype=np.float64)
+    rows = np.arange(n)
+    dpos = ia[:-1] + cl
+    ja[dpos] = rows
+    a[dpos] = np.where(sums > 0, diag_factor * sums, 1.0)
+    # upper entries, already in (row, column) order
+    ustart = np.zeros(n, dtype=np.int64)
+    np.cumsum(cu[:-1], out=ustart[1:])
+    upos = dpos[lo] + 1 + (np.arange(lo.shape[0]) - ustart[lo])
+    ja[upos] = hi
+    a[upos] = -ew
+    # lower entries: the transpose, ordered by (hi, lo)
+    order = np.argsort(hi * n + lo)  # keys are distinct: a unique order
+    lr, lc = hi[order], lo[order]
+    lstart = np.zeros(n, dtype=np.int64)
+    np.cumsum(cl[:-1], out=lstart[1:])
+    lpos = ia[lr] + (np.arange(lr.shape[0]) - lstart[lr])
+    ja[lpos] = lc
+    a[lpos] = -ew[order]
+    return CSR(n, ia.astype(np.int32), ja, a)

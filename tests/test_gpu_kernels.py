@@ -9,7 +9,7 @@ import torch
 
 from cases import PROBLEMS, star_graph
 from lap_amd import solver as S
-from lap_amd.problems import CSR, augment_lower, augment_upper, grid_laplacian, split
+from lap_amd.problems import CSR, augment_lower, augment_upper, chung_lu, grid_laplacian, split
 
 pytestmark = pytest.mark.gpu
 
@@ -272,4 +272,39 @@ def test_trisolve_sync_free_large_bitwise(orc, sync_free):
     """A 48^3 grid (142 levels, ~1700 chunks): every dependency crosses
     chunks claimed by other waves."""
     A = grid_laplacian([48, 48, 48], 1e-8)
+    test_trisolve_bitwise(orc, A)
+
+
+LONG_ROWS = {"power_law": lambda: chung_lu(16384, seed=7), "star": lambda: star_graph(6000)}
+
+
+@pytest.mark.parametrize("name", sorted(LONG_ROWS))
+def test_trisolve_long_rows_bitwise(orc, name):
+    """Rows of more than 32 entries take a workgroup each (the power-law
+    graph's U rows reach 5417 entries: three LDS chunks; the star's hub row
+    5999); the subtractions keep the sequential order, so the solves, the
+    IC0 factor and the IC0 preconditioner stay bitwise.  GS_std's backward
+    sweeps start from r - A w, whose rows of more than 2048 entries are
+    summed in tree order: there the result is within a few ulps of the
+    terms' magnitudes."""
+    A = LONG_ROWS[name]()
+    print(f"[case] long rows {name} n={A.n} max row {np.diff(A.ia).max()}", flush=True)
+    test_trisolve_bitwise(orc, A)
+    test_ichol_bitwise(orc, A)
+    test_prec_bitwise(orc, A, "ichol")
+    r = np.random.default_rng(6).standard_normal(A.n)
+    D = S.DeviceCSR.from_csr(A)
+    for k in (1, 2):
+        P = S.make_preconditioner(D, A, "GS_std", m=k, k=k)
+        got = host(S.prec_apply(D, P, dev(r)))
+        want = orc.prec_apply(A, orc.make_prec(A, "GS_std", m=k, k=k), r)
+        if np.diff(A.ia).max() <= 2048:
+            np.testing.assert_array_equal(got, want)
+        else:
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_trisolve_long_rows_one_launch_bitwise(orc, sync_free):
+    A = LONG_ROWS["power_law"]()
+    print(f"[case] long rows one-launch n={A.n}", flush=True)
     test_trisolve_bitwise(orc, A)

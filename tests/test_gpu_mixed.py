@@ -7,7 +7,13 @@ tolerance allows (relative error <= 1e3 * tol for these well-conditioned
 cases); inner iteration count within 25% of the FP64 solver's with a
 preconditioner, 50% without one (FP32 loses orthogonality faster there).
 
-IC0 is tested on the graph with a 1e-3 shift only: on a Laplacian shifted by
+The power-law graph (Chung-Lu, hub rows of ~5000 lognormal weights, diagonal
+(1 + 1e-6) times the weight sum) is the case plain FP32 rounding of the
+off-diagonals fails on: a hub row's sum moves by more than the 1e-6 shift
+and the FP32 operator turns indefinite.  With the rounding error of each
+row folded into its FP64 diagonal the inner operator keeps A's row sums.
+
+IC0 is tested on the graphs with the larger shifts only: on a Laplacian shifted by
 1e-8 the FP32 rounding of the residual puts ~6e-8 relative noise into the
 near-null constant mode, which an IC0 preconditioner (unlike Jacobi)
 amplifies by ~1e8, and the FP32 inner process stagnates."""
@@ -17,7 +23,7 @@ import pytest
 from cases import PROBLEMS
 from lap_amd import lib as L
 from lap_amd import solver as S
-from lap_amd.problems import grid_laplacian, rhs_normal
+from lap_amd.problems import chung_lu, grid_laplacian, rhs_normal
 
 pytestmark = pytest.mark.gpu
 
@@ -26,12 +32,16 @@ def centred(x):
     return x - x.mean()
 
 
+CASES = dict(PROBLEMS, power_law=lambda: chung_lu(16384, seed=7))
+
+
 @pytest.mark.parametrize("kind,pname", [("line_jacobi", "grid3d_16"), ("none", "grid3d_16"),
                                         ("line_jacobi", "graph_big"), ("ichol", "graph_big"),
-                                        ("none", "graph_big")])
+                                        ("none", "graph_big"), ("line_jacobi", "power_law"),
+                                        ("ichol", "power_law")])
 def test_mixed_converges_to_fp64_tolerance(orc, kind, pname):
     print(f"[case] mixed kind={kind} problem={pname}", flush=True)
-    Ah = PROBLEMS[pname]()
+    Ah = CASES[pname]()
     b = rhs_normal(Ah.n, 31)
     A = S.DeviceCSR.from_csr(Ah)
     P = S.make_preconditioner(A, Ah, kind)
@@ -48,7 +58,11 @@ def test_mixed_converges_to_fp64_tolerance(orc, kind, pname):
           f"{err:.2e}", flush=True)
     assert m.flag == 0
     assert true < tol * r0 * 1.01
-    assert m.iterations <= (1.5 if kind == "none" else 1.25) * d.iterations + 5
+    # FP32 inner vectors: conjugacy decays at the FP32 rounding level, which on
+    # the power-law graph (Jacobi-scaled condition ~1e6) costs 2.7-3x the
+    # FP64 iterations; the grids and the 1e-3-shifted graph stay close
+    slack = 3.5 if pname == "power_law" else 1.5 if kind == "none" else 1.25
+    assert m.iterations <= slack * d.iterations + 5
     assert err <= 1e3 * tol
 
 
@@ -75,3 +89,38 @@ def test_mixed_rejects_unsupported_preconditioner():
     P = S.make_preconditioner(A, Ah, "GS_std")
     with pytest.raises(L.LapError):
         S.pcg_mixed(A, S.to_dev(rhs_normal(Ah.n, 1)), P)
+
+
+@pytest.fixture
+def fp64_inner_vectors():
+    prev = S.set_option("mixed_vectors", 1)
+    yield
+    S.set_option("mixed_vectors", prev)
+
+
+@pytest.mark.parametrize("kind", ["line_jacobi", "ichol"])
+def test_mixed_power_law_fp64_vectors(orc, kind, fp64_inner_vectors):
+    """FP32 matrix values (row sums kept) with FP64 inner vectors: the inner
+    process is FP64 CG on an operator within a factor 1 +- 6e-8 of A, so on
+    the power-law graph (whose FP32-vector inner process needs 2-4x the
+    iterations) it takes the FP64 solve's iteration count, give or take a
+    few, with one refinement per two decades of residual."""
+    print(f"[case] mixed fp64 vectors kind={kind} power_law", flush=True)
+    Ah = CASES["power_law"]()
+    b = rhs_normal(Ah.n, 31)
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, kind)
+    m = S.pcg_mixed(A, S.to_dev(b), P, tol=1e-10, maxit=5000)
+    S.set_option("mixed_vectors", 0)
+    d = S.pcg(A, S.to_dev(b), P, tol=1e-10, maxit=5000)
+    x = m.x.cpu().numpy()
+    true = np.linalg.norm(b - orc.matvec(Ah, x))
+    xo, ito, _, _ = orc.cg(Ah, b, kind, tol=1e-10, maxit=5000)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"[diag] mixed fp64-vector {kind}/power_law: it {m.iterations} (fp64 {d.iterations}), "
+          f"refinements {m.refinements}, true rel res {true / np.linalg.norm(b):.2e}, err vs "
+          f"oracle {err:.2e}", flush=True)
+    assert m.flag == 0 and true < 1e-10 * np.linalg.norm(b) * 1.01
+    assert m.iterations <= d.iterations + 5
+    assert m.refinements <= 6
+    assert err <= 1e3 * 1e-10
