@@ -266,6 +266,82 @@ def bench_energy(S, torch, A, b, P, tol: float, solves: int = 3) -> dict:
     return out
 
 
+def bench_power_law(S, torch, np, tol: float, n: int = 1 << 22, reps: int = 2,
+                    kinds=("line_jacobi", "ichol")) -> dict:
+    """The mixed-precision workload on a power-law graph: weighted Chung-Lu
+    Laplacian, n = 2^22, degree exponent 2.1, mean degree 16, degrees capped
+    at 20000, lognormal(0, 1) weights, diagonal (1 + 1e-6) x the weight sum;
+    b ~ N(0,1) (seed 12345) mean-subtracted.  FP64 PCG against the
+    mixed-precision solve (FP32 matrix values with row sums kept, FP64
+    refinement to `tol`) with FP64 inner vectors (mixed_vectors 1) and, for
+    Jacobi, FP32 inner vectors (mixed_vectors 0); Jacobi and IC0.  Also the
+    fused SpMV's rate on this matrix (gather-bound: its columns are random
+    beyond the hub prefix)."""
+    from lap_amd.problems import chung_lu, rhs_normal
+
+    t0 = time.perf_counter()
+    Ah = chung_lu(n)
+    out = {"workload": ("weighted Chung-Lu Laplacian n=2^22, gamma 2.1, mean degree 16, cap "
+                        "20000, lognormal(0,1) weights, diag (1+1e-6)*sum w; b~N(0,1) seed "
+                        f"12345 mean-subtracted; PCG to rel-res {tol:g}"),
+           "n": n, "nnz": Ah.nnz, "max_row": int(np.diff(Ah.ia).max()),
+           "build_s": time.perf_counter() - t0}
+    A = S.DeviceCSR.from_csr(Ah)
+    b = S.to_dev(rhs_normal(n, 12345))
+    x = torch.randn(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    ms = S.spmv_timed(A, x, y, 20, fused=True)
+    gbs = spmv_bytes(n, Ah.nnz) / (ms * 1e-3) / 1e9
+    out["spmv"] = {"kernel": "k_spmv_pq", "avg_ms": ms, "achieved_gbs": gbs,
+                   "frac": gbs / HBM_PEAK_GBS}
+    del x, y
+
+    def timed(fn, count):
+        res, t = [], time.perf_counter()
+        for _ in range(count):
+            res.append(fn())
+        torch.cuda.synchronize()
+        return res, (time.perf_counter() - t) / count
+
+    prev = S.set_option("mixed_vectors", 0)
+    try:
+        for kind in kinds:
+            count = reps if kind == "line_jacobi" else 1
+            t = time.perf_counter()
+            P = S.make_preconditioner(A, Ah, kind)
+            torch.cuda.synchronize()
+            rec = {"setup_s": time.perf_counter() - t}
+            fp64 = lambda: S.pcg(A, b, P, tol=tol, maxit=25000)
+            if count > 1:
+                fp64()
+            res, wall = timed(fp64, count)
+            d = res[-1]
+            rec["fp64"] = {"s": wall, "iterations": d.iterations, "flag": d.flag,
+                           "device_ms": d.gpu_ms, "true_rel_res": d.true_res / d.res0}
+            log(f"power_law {kind} fp64: {d.iterations} iterations {wall:.3f} s")
+            for mv in ((1, 0) if kind == "line_jacobi" else (1,)):
+                S.set_option("mixed_vectors", mv)
+                mixed = lambda: S.pcg_mixed(A, b, P, tol=tol, maxit=25000)
+                if count > 1:
+                    mixed()
+                res, mwall = timed(mixed, count)
+                m = res[-1]
+                err = float(torch.linalg.norm(m.x - d.x) / torch.linalg.norm(d.x))
+                rec[f"mixed_vectors_{mv}"] = {
+                    "s": mwall, "iterations": m.iterations, "refinements": m.refinements,
+                    "flag": m.flag, "device_ms": m.gpu_ms, "true_rel_res": m.true_res / m.res0,
+                    "speedup_vs_fp64": wall / mwall, "rel_err_vs_fp64_solution": err}
+                log(f"power_law {kind} mixed_vectors {mv}: {m.iterations} iterations "
+                    f"{mwall:.3f} s ({wall / mwall:.2f}x)")
+            S.set_option("mixed_vectors", 0)
+            out[kind] = rec
+            del P
+    finally:
+        S.set_option("mixed_vectors", prev)
+    A.forget()
+    return out
+
+
 def bench_value_indexed(S, torch, A, b, P, tol: float, ref, reps: int) -> dict:
     """The headline solve with the SpMV reading value-indexed CSR (option
     spmv_format 1: one-byte indices into the matrix's 5 distinct doubles,
@@ -513,7 +589,9 @@ def main() -> int:
                          "measurements")
     ap.add_argument("--only", default="",
                     help="comma-separated extras to run (preconditioners, value_indexed, energy, "
-                         "sweeps, block, batched, adjoint, input); default all")
+                         "power_law, sweeps, block, batched, adjoint, input); default all")
+    ap.add_argument("--power-law-kinds", default="line_jacobi,ichol",
+                    help="preconditioners of the power_law extra")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the kernels the HBM-traffic counters are read on: a 1 GiB "
                          "device copy (calibration) and the fused SpMV, then exit")
@@ -645,6 +723,8 @@ def main() -> int:
                           lambda: bench_value_indexed(S, torch, A, b, P, args.tol, results[-1],
                                                       args.spmv_reps)),
                          ("energy", lambda: bench_energy(S, torch, A, b, P, args.tol)),
+                         ("power_law", lambda: bench_power_law(
+                             S, torch, np, args.tol, kinds=args.power_law_kinds.split(","))),
                          ("sweeps", lambda: bench_sweeps(S, torch, A, Ah, b, args.tol)),
                          ("block", lambda: bench_block(S, torch, np)),
                          ("batched", lambda: bench_batched(S, torch, np)),

@@ -108,6 +108,30 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// s (+ or -) p[k0] (+ or -) p[k0 + 1] ... (+ or -) p[k1 - 1], one term at a
+// time in that order: the sequential loop's result bit for bit.  The loads
+// (p is LDS) are issued eight at a time ahead of the dependent adds, so the
+// chain pays the add latency rather than a load round trip per term.
+template <bool SUB = false>
+__device__ __forceinline__ double lds_seq_sum(double s, const double *p, int k0, int k1) {
+  int k = k0;
+  for (; k + 8 <= k1; k += 8) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = p[k + j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s = SUB ? s - v[j] : s + v[j];
+  }
+  const int m = k1 - k;  // 0..7 left
+  double v[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) v[j] = j < m ? p[k + j] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+    if (j < m) s = SUB ? s - v[j] : s + v[j];
+  return s;
+}
+
 // Sum over a block of BS threads; the result is valid in thread 0.
 // Fixed summation order, so deterministic.
 template <int BS>

@@ -46,6 +46,7 @@ __device__ __forceinline__ void spmv_long_row(int r0, int e0, int e1, const int 
                                               const XT *__restrict__ x, double alpha,
                                               double *red, Epi &epi) {
   double s = 0.0;
+#pragma unroll 4
   for (int e = e0 + (int)threadIdx.x; e < e1; e += BS)
     s += (alpha * (double)a[e]) * (double)x[ja[e]];
   const double t = block_sum<BS>(s, red);
@@ -59,9 +60,7 @@ __device__ __forceinline__ void spmv_row_sums(int r0, int r1, int e0, const int 
                                               const double *prod, Epi &epi) {
   for (int row = r0 + (int)threadIdx.x; row < r1; row += BS) {
     const int s0 = ia[row] - e0, s1 = ia[row + 1] - e0;
-    double s = epi.init(row);
-    for (int k = s0; k < s1; ++k) s += prod[k];
-    epi.store(row, s);
+    epi.store(row, lds_seq_sum(epi.init(row), prod, s0, s1));
   }
 }
 
@@ -118,9 +117,7 @@ __device__ __forceinline__ void spmv_block_rows_pf(int b, const int *__restrict_
   __syncthreads();
   for (int row = r0 + (int)threadIdx.x; row < r1; row += BS) {
     const int s0 = rp[row - r0], s1 = rp[row - r0 + 1];
-    double s = epi.init(row);
-    for (int k = s0; k < s1; ++k) s += prod[k];
-    epi.store(row, s);
+    epi.store(row, lds_seq_sum(epi.init(row), prod, s0, s1));
   }
   __syncthreads();
 }

@@ -51,10 +51,10 @@ namespace {
 // One launch per level.  The level's short rows (the first `cnt` records)
 // take one lane each; each of its `nlong` long rows (the records after them)
 // takes a workgroup of its own: the workgroup forms the row's products a
-// chunk at a time into LDS and lane 0 subtracts them in CSR order (the
-// pivot's slot holds +0.0, and x - (+0.0) == x bit for bit), so a long row
-// gets exactly the lane path's arithmetic while its loads proceed in
-// parallel.  AT/VT (value/vector type) = double/double: the FP64 solve;
+// chunk at a time into LDS (two buffers: waves 1.. form the next chunk while
+// lane 0 subtracts the current one in CSR order; the pivot's slot holds
+// +0.0, and x - (+0.0) == x bit for bit), so a long row gets exactly the
+// lane path's arithmetic while its loads proceed in parallel.  AT/VT (value/vector type) = double/double: the FP64 solve;
 // float/float or float/double: FP32 values with FP64 products and sums (the
 // mixed-precision inner solve).
 constexpr int kTriChunk = 2048;
@@ -91,29 +91,54 @@ __global__ __launch_bounds__(kBlock) void k_tri_level(
   }
   const int lr = (int)blockIdx.x - gshort;
   if (lr >= nlong) return;
-  __shared__ double prod[kTriChunk];
+  __shared__ double prod[2][kTriChunk];
   __shared__ int dpos;
   const int4 d = desc[cnt + lr];
   const int row = d.x;
+  const int nch = (d.z - d.y + kTriChunk - 1) / kTriChunk;
   if (threadIdx.x == 0) dpos = -1;
+  __syncthreads();
   double acc = (double)x[row];
-  for (int base = d.y; base < d.z; base += kTriChunk) {
+  // threads t0 .. t0 + nt - 1 form chunk ch's products, loads issued in
+  // batches of four ahead of their use
+  auto fill = [&](int ch, int t0, int nt) {
+    const int base = d.y + ch * kTriChunk;
     const int m = min(kTriChunk, d.z - base);
-    __syncthreads();  // the previous chunk has been consumed
-    for (int k = threadIdx.x; k < m; k += kBlock) {
-      const int e = base + k;
-      const int c = ja[e];
-      if (c == row) {
-        prod[k] = 0.0;
-        atomicMax(&dpos, e);  // the last pivot entry, as the lane path keeps
-      } else {
-        prod[k] = (double)a[e] * (double)y[c];
+    double *buf = prod[ch & 1];
+    for (int k0 = (int)threadIdx.x - t0; k0 < m; k0 += 4 * nt) {
+      int c[4];
+      AT av[4];
+      VT yv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = k0 + j * nt;
+        c[j] = k < m ? ja[base + k] : row;
+        av[j] = k < m ? a[base + k] : AT(0);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) yv[j] = c[j] != row ? y[c[j]] : VT(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = k0 + j * nt;
+        if (k >= m) continue;
+        if (c[j] == row) {
+          buf[k] = 0.0;
+          atomicMax(&dpos, base + k);  // the last pivot entry, as the lane path keeps
+        } else {
+          buf[k] = (double)av[j] * (double)yv[j];
+        }
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll 8
-      for (int k = 0; k < m; ++k) acc -= prod[k];
+  };
+  fill(0, 0, kBlock);
+  // chunk ch is subtracted by lane 0 while waves 1.. form chunk ch + 1
+  for (int ch = 0; ch < nch; ++ch) {
+    __syncthreads();  // chunk ch formed, chunk ch - 1 consumed
+    if (threadIdx.x < kWave) {
+      if (threadIdx.x == 0)
+        acc = lds_seq_sum<true>(acc, prod[ch & 1], 0, min(kTriChunk, d.z - d.y - ch * kTriChunk));
+    } else if (ch + 1 < nch) {
+      fill(ch + 1, kWave, kBlock - kWave);
     }
   }
   __syncthreads();

@@ -91,9 +91,7 @@ __device__ double row_sum_any(int e0, int e1, const int *__restrict__ ja,
     for (int k = threadIdx.x; k < cnt; k += BS) prod[k] = (1.0 * a[e0 + k]) * x[ja[e0 + k]];
     __syncthreads();
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int k = 0; k < cnt; ++k) s += prod[k];
-      red[0] = s;
+      red[0] = lds_seq_sum(0.0, prod, 0, cnt);
     }
   }
   __syncthreads();
@@ -131,11 +129,10 @@ __global__ __launch_bounds__(BS) void k_sweep_pair(
       prod[cl + k] = (1.0 * ua[u0 + k]) * x[uja[u0 + k]];
     __syncthreads();
     for (int row = r0 + (int)threadIdx.x; row < r1; row += BS) {
-      double sl = 0.0, su = 0.0;
       const int a0 = lia[row] - l0, a1 = lia[row + 1] - l0;
-      for (int k = a0; k < a1; ++k) sl += prod[k];
+      const double sl = lds_seq_sum(0.0, prod, a0, a1);
       const int c0 = uia[row] - u0 + cl, c1 = uia[row + 1] - u0 + cl;
-      for (int k = c0; k < c1; ++k) su += prod[k];
+      const double su = lds_seq_sum(0.0, prod, c0, c1);
       out[row] = (base[row] - (su + sl)) * d_r[row];
     }
     __syncthreads();
