@@ -388,6 +388,56 @@ def bench_value_indexed(S, torch, A, b, P, tol: float, ref, reps: int) -> dict:
         "spmv_roofline": {"bound": "hbm", "kernel": "k_spmv_pq_vi", "achieved": ach,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                           "algorithmic_bytes": vib, "avg_ms": ms},
+        "delta_indexed": bench_delta_indexed(S, torch, A, b, P, tol, ref, reps),
+    }
+
+
+def bench_delta_indexed(S, torch, A, b, P, tol: float, ref, reps: int) -> dict:
+    """The headline solve with the SpMV reading delta-indexed CSR (option
+    spmv_format 2: per nonzero a 16-bit code, one byte indexing the 5
+    distinct values, one the 7 distinct column offsets col - row; 2 B per
+    nonzero).  Iterates bitwise the CSR solve's; the SpMV's roofline uses its
+    own algorithmic bytes, 2*nnz + 4(n+1) + 16n.  The mixed-precision solve
+    with its FP32 off-diagonal copy delta-indexed is timed too."""
+    try:
+        S.set_option("spmv_format", 2)
+        S.pcg(A, b, P, tol=tol, maxit=25000)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = S.pcg(A, b, P, tol=tol, maxit=25000)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        x = torch.randn(A.n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        ms = S.spmv_timed(A, x, y, reps, fused=True)
+        S.pcg_mixed(A, b, P, tol=tol, maxit=25000)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m = S.pcg_mixed(A, b, P, tol=tol, maxit=25000)
+        torch.cuda.synchronize()
+        mwall = time.perf_counter() - t0
+    finally:
+        S.set_option("spmv_format", 0)
+    dib = 2 * A.nnz + 4 * (A.n + 1) + 16 * A.n
+    ach = dib / (ms * 1e-3) / 1e9
+    return {
+        "value": wall, "unit": "s", "iterations": r.iterations, "flag": r.flag,
+        "device_ms_per_solve": r.gpu_ms,
+        "ms_per_iteration": r.gpu_ms / max(r.iterations, 1),
+        "bitwise_equal_to_csr_solve": bool(torch.equal(r.x, ref.x)
+                                           and r.iterations == ref.iterations),
+        "speedup_vs_csr": ref.gpu_ms / r.gpu_ms,
+        "mixed_precision": {
+            "value": mwall, "unit": "s", "iterations": m.iterations,
+            "refinements": m.refinements, "flag": m.flag, "device_ms_per_solve": m.gpu_ms,
+            "true_rel_res": m.true_res / m.res0,
+            "speedup_vs_fp64_csr": ref.gpu_ms / m.gpu_ms,
+            "speedup_vs_fp64_delta_indexed": r.gpu_ms / m.gpu_ms,
+            "scheme": "the mixed_precision solve with its FP32 off-diagonal copy delta-indexed",
+        },
+        "spmv_roofline": {"bound": "hbm", "kernel": "k_spmv_pq_dvi", "achieved": ach,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                          "algorithmic_bytes": dib, "avg_ms": ms},
     }
 
 

@@ -192,8 +192,8 @@ int lap_set_option(const char *name, int value) {
     set_error(LAP_ERR_ARG, "mixed_vectors must be 0 or 1");
     return *slot;
   }
-  if (!strcmp(name, "spmv_format") && (value < 0 || value > 1)) {
-    set_error(LAP_ERR_ARG, "spmv_format must be 0 or 1");
+  if (!strcmp(name, "spmv_format") && (value < 0 || value > 2)) {
+    set_error(LAP_ERR_ARG, "spmv_format must be 0, 1 or 2");
     return *slot;
   }
   if (!strcmp(name, "fuse_sweeps") && (value < 0 || value > 1)) {

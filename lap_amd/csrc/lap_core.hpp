@@ -234,7 +234,9 @@ struct Options {
   int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
   int mixed_vectors = 0;   // lap_pcg_mixed inner vectors: 0 FP32, 1 FP64
   int spmv_format = 0;     // PCG SpMV: 0 CSR values, 1 value-indexed when the
-                           // matrix has at most 255 distinct values
+                           // matrix has at most 255 distinct values, 2 also
+                           // the columns as offset indices (<= 255 distinct
+                           // col - row, rows <= 64 entries), else as 1
   int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
                            // 1 three kernels, x in K_C; 2 two kernels (p in the
                            // SpMV gather)
@@ -275,6 +277,13 @@ unsigned spmv_grid(const SpmvPlan &p);
 // (at most 255, bit patterns ascending).  Returns the table size, 0 when the
 // values do not fit, -1 after an error.
 int vi_encode(long long nnz, const double *a, unsigned char *vi, double *dict);
+// Delta-indexed CSR: one 16-bit code per nonzero, low byte = index into
+// vdict (the distinct values, as vi_encode), high byte = index into ddict
+// (the distinct column offsets col - row, ascending as unsigned).  Returns 1
+// when encoded, 0 when either table would exceed 255 entries or a row is
+// longer than max_row, -1 after an error.
+int dvi_encode(int n, long long nnz, const int *ia, const int *ja, const double *a, int max_row,
+               unsigned short *code, double *vdict, int *ddict, int *nv, int *ndl);
 
 // --------------------------------------------------- SpMV-only sweeps
 // Row blocks over a pair of CSR matrices (the strict L and U of a split),

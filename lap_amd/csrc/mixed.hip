@@ -234,6 +234,71 @@ __global__ __launch_bounds__(BS) void k_mx_spmv_pq_vi(int nblk, const int *__res
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// The same over the delta-indexed copy: code = v | d << 8, value dict[v],
+// column row + ddict[d] (tables in LDS); the block's row pointers and codes
+// are staged into LDS with coalesced loads, then one lane per row (rows of at
+// most 64 entries) sums its products in CSR order after the diagonal term:
+// bitwise the CSR kernel's rows.
+template <int BS, class VT>
+__global__ __launch_bounds__(BS) void k_mx_spmv_pq_dvi(int nblk, const int *__restrict__ rowblk,
+                                                       const int *__restrict__ ia,
+                                                       const unsigned short *__restrict__ code,
+                                                       const float *__restrict__ dict, int nd,
+                                                       const int *__restrict__ ddict, int ndl,
+                                                       const double *__restrict__ diag,
+                                                       const VT *__restrict__ p,
+                                                       VT *__restrict__ q, MixCtl *ctl,
+                                                       double *part, int walk) {
+  if (ctl->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->pend_valid) {
+    ctl->iter = ctl->pend_iter;
+    ctl->rho = ctl->pend_rho;
+    ctl->maxr2 = ctl->pend_maxr2;
+    ctl->refinements = ctl->pend_refinements;
+    ctl->pend_valid = 0;
+  }
+  __shared__ double red[BS / kWave];
+  __shared__ float vt[256];
+  __shared__ int dt[256];
+  __shared__ unsigned short sc[kSpmvTile];
+  __shared__ int rp[2 * BS + 1];
+  for (int i = threadIdx.x; i < nd; i += BS) vt[i] = dict[i];
+  for (int i = threadIdx.x; i < ndl; i += BS) dt[i] = ddict[i];
+  PqMxEpi<VT> epi{q, p, diag, 0.0};
+  const BlockWalk w = block_walk(nblk, walk);
+  for (int s = w.begin; s < w.end; s += w.step) {
+    const int b = w.block(s);
+    if (b >= nblk) continue;
+    const int r0 = rowblk[b], r1 = rowblk[b + 1];
+    const int e0 = ia[r0];
+    for (int i = threadIdx.x; i <= r1 - r0; i += BS) rp[i] = ia[r0 + i] - e0;
+    __syncthreads();
+    const int cnt = rp[r1 - r0];
+    for (int k = threadIdx.x; k < cnt; k += BS) sc[k] = code[e0 + k];
+    __syncthreads();
+    for (int row = r0 + (int)threadIdx.x; row < r1; row += BS) {
+      const int k1 = rp[row - r0 + 1];
+      double sum = epi.init(row);
+      for (int k = rp[row - r0]; k < k1; k += 8) {
+        unsigned c[8];
+        double xv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) c[j] = k + j < k1 ? (unsigned)sc[k + j] : 0u;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          xv[j] = k + j < k1 ? (double)p[row + dt[c[j] >> 8]] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (k + j < k1) sum += (1.0 * (double)vt[c[j] & 255u]) * xv[j];
+      }
+      epi.store(row, sum);
+    }
+    __syncthreads();
+  }
+  const double t = block_sum<BS>(epi.acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // alpha = rho/(p.q); x32 += alpha p; r32 -= alpha q (each rounded once from
 // FP64 to VT); partials of r.r and, with Jacobi, r.w for w = d_r r (stride
 // 2: [rz, rr]).
@@ -453,6 +518,11 @@ struct MixPlan {
   // one byte per off-diagonal entry and the table of the values
   DevBufPtr vi_o, dict_o;
   int nd = 0;
+  // delta-indexed copy (when also the column offsets col - row take at most
+  // 255 values and rows hold at most 64 entries): code = value index |
+  // offset index << 8
+  DevBufPtr code_o, ddict_o;
+  int ndl = 0;
   const SpmvPlan *off = nullptr;
   std::map<const double *, DevBufPtr> dr32;  // by d_r pointer
   std::map<const IcPlan *, std::pair<DevBufPtr, DevBufPtr>> ic32;
@@ -511,6 +581,26 @@ MixPlan *mix_plan(int n, int nnz, const int *ia, const int *ja, const double *a)
       P->vi_o = dev_upload(vi);
       P->dict_o = dev_upload(dict);
       P->nd = (int)keys.size();
+      // the offsets
+      std::vector<int> deltas;
+      int longest = 0;
+      for (int i = 0; i < n; ++i) {
+        longest = std::max(longest, io[i + 1] - io[i]);
+        for (int e = io[i]; e < io[i + 1]; ++e) deltas.push_back(jo[e] - i);
+      }
+      std::vector<int> dk(deltas);
+      std::sort(dk.begin(), dk.end());
+      dk.erase(std::unique(dk.begin(), dk.end()), dk.end());
+      if (longest <= 64 && !dk.empty() && dk.size() <= 255) {
+        std::vector<unsigned short> code(ao.size());
+        for (size_t e = 0; e < ao.size(); ++e)
+          code[e] = (unsigned short)(vi[e] | (std::lower_bound(dk.begin(), dk.end(), deltas[e]) -
+                                              dk.begin())
+                                                 << 8);
+        P->code_o = dev_upload(code);
+        P->ddict_o = dev_upload(dk);
+        P->ndl = (int)dk.size();
+      }
     }
   }
   P->off = spmv_plan(n, P->nnz_off, P->ia_o->as<int>(), P->ja_o->as<int>());
@@ -660,9 +750,14 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   const float *ao = M->a32_o->as<float>();
   const double *dg = M->diag->as<double>();
   // value-indexed inner SpMV when spmv_format asks for it and the table fits
-  const bool vi32 = options().spmv_format == 1 && M->nd > 0;
+  const bool dvi32 = options().spmv_format == 2 && M->ndl > 0;
+  const bool vi32 = !dvi32 && options().spmv_format >= 1 && M->nd > 0;
   auto spmv32 = [&]() {
-    if (vi32)
+    if (dvi32)
+      k_mx_spmv_pq_dvi<kBlock, VT><<<gO, kBlock, 0, stream()>>>(
+          M->off->nblocks, rb_o, io, M->code_o->as<unsigned short>(), M->dict_o->as<float>(),
+          M->nd, M->ddict_o->as<int>(), M->ndl, dg, p32, q32, ctl, Q1, walk);
+    else if (vi32)
       k_mx_spmv_pq_vi<kBlock, kSpmvTile, VT><<<gO, kBlock, 0, stream()>>>(
           M->off->nblocks, rb_o, io, jo, M->vi_o->as<unsigned char>(), M->dict_o->as<float>(),
           M->nd, dg, p32, q32, ctl, Q1, walk);

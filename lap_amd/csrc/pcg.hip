@@ -224,18 +224,112 @@ __global__ __launch_bounds__(BS) void k_spmv_pq_vi(int nblk, const int *__restri
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// A value-indexed copy of the matrix values for one solve (vi == nullptr:
-// the CSR values are used).
+constexpr int kDviTile = kSpmvTile;  // codes staged per row block
+
+// K_A over delta-indexed CSR: nonzero e of row r holds the 16-bit code
+// c = v | d << 8, value vt[v] and column r + dt[d] (both tables in LDS).  A
+// row block's row pointers and codes are staged into LDS with coalesced
+// loads; then one lane per row (rows of at most 64 entries) decodes its
+// entries eight at a time and sums the products (1.0 * value) * p[col] in
+// CSR order, so each row is bitwise the CSR kernel's.  2 bytes per nonzero
+// instead of 12.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_spmv_pq_dvi(int nblk, const int *__restrict__ rowblk,
+                                                    const int *__restrict__ ia,
+                                                    const unsigned short *__restrict__ code,
+                                                    const double *__restrict__ vdict, int nv,
+                                                    const int *__restrict__ ddict, int ndl,
+                                                    const double *__restrict__ p,
+                                                    double *__restrict__ q, PcgCtl *ctl,
+                                                    double *partials, int xcd) {
+  if (ctl->done) return;
+  if (ctl->pend_done) {  // the previous kernel stopped the solve: commit it
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = 1;
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->pend_valid) {
+    ctl->iter = ctl->pend_iter;
+    ctl->rho = ctl->pend_rho;
+    ctl->pend_valid = 0;
+  }
+  __shared__ double red[BS / kWave];
+  __shared__ double vt[256];
+  __shared__ int dt[256];
+  __shared__ unsigned short sc[kDviTile];  // the block's codes
+  __shared__ int rp[2 * BS + 1];           // its row pointers, relative to e0
+  for (int i = threadIdx.x; i < nv; i += BS) vt[i] = vdict[i];
+  for (int i = threadIdx.x; i < ndl; i += BS) dt[i] = ddict[i];
+  PqEpi epi{q, p, 0.0};
+  const BlockWalk w = block_walk(nblk, xcd);
+  for (int s = w.begin; s < w.end; s += w.step) {
+    const int b = w.block(s);
+    if (b >= nblk) continue;
+    const int r0 = rowblk[b], r1 = rowblk[b + 1];
+    const int e0 = ia[r0];
+    // coalesced staging: row pointers, then the codes of the block
+    for (int i = threadIdx.x; i <= r1 - r0; i += BS) rp[i] = ia[r0 + i] - e0;
+    __syncthreads();
+    const int cnt = rp[r1 - r0];
+    for (int k = threadIdx.x; k < cnt; k += BS) sc[k] = code[e0 + k];
+    __syncthreads();
+    for (int row = r0 + (int)threadIdx.x; row < r1; row += BS) {
+      const int k1 = rp[row - r0 + 1];
+      double sum = epi.init(row);
+      for (int k = rp[row - r0]; k < k1; k += 8) {
+        unsigned c[8];
+        double xv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) c[j] = k + j < k1 ? (unsigned)sc[k + j] : 0u;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) xv[j] = k + j < k1 ? p[row + dt[c[j] >> 8]] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (k + j < k1) sum += (1.0 * vt[c[j] & 255u]) * xv[j];
+      }
+      epi.store(row, sum);
+    }
+    __syncthreads();
+  }
+  const double t = block_sum<BS>(epi.acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// A value-indexed copy of the matrix for one solve (vi == nullptr and
+// code == nullptr: the CSR arrays are used).
 struct ViVals {
   const unsigned char *vi = nullptr;
   const double *dict = nullptr;
   int nd = 0;
+  const unsigned short *code = nullptr;  // delta-indexed (dict = value table)
+  const int *ddict = nullptr;
+  int ndl = 0;
 };
 
-// Encodes a when options().spmv_format asks for it and the values fit.
-static ViVals vi_prepare(int nnz, const double *a) {
+constexpr int kDviMaxRow = 64;
+
+// Encodes the matrix when options().spmv_format asks for it and it fits:
+// format 2 tries the delta-indexed form first, then (like format 1) the
+// value-indexed one.
+static ViVals vi_prepare(int nnz, const double *a, int n = 0, const int *ia = nullptr,
+                         const int *ja = nullptr) {
   ViVals v;
-  if (options().spmv_format != 1) return v;
+  const int fmt = options().spmv_format;
+  if (fmt == 2 && ia && ja && n > 0) {
+    unsigned short *code =
+        reinterpret_cast<unsigned short *>(scratch_vec("dvi_code", ((size_t)nnz + 3) / 4 + 1));
+    double *vd = scratch_vec("dvi_vdict", 256);
+    int *dd = reinterpret_cast<int *>(scratch_vec("dvi_ddict", 128));
+    int nv = 0, ndl = 0;
+    if (dvi_encode(n, nnz, ia, ja, a, kDviMaxRow, code, vd, dd, &nv, &ndl) == 1) {
+      v.code = code;
+      v.dict = vd;
+      v.nd = nv;
+      v.ddict = dd;
+      v.ndl = ndl;
+      return v;
+    }
+  }
+  if (fmt != 1 && fmt != 2) return v;
   unsigned char *vi =
       reinterpret_cast<unsigned char *>(scratch_vec("vi_idx", ((size_t)nnz + 7) / 8 + 1));
   double *dict = scratch_vec("vi_dict", 256);
@@ -642,6 +736,11 @@ static void launch_spmv_pq(unsigned g, const SpmvPlan &A, const int *ia, const i
                            double *part, const ViVals &vv = ViVals()) {
   const int xcd = options().spmv_walk;
   const int *rb = A.rowblk->as<int>();
+  if (vv.code) {
+    k_spmv_pq_dvi<kBlock><<<g, kBlock, 0, stream()>>>(A.nblocks, rb, ia, vv.code, vv.dict, vv.nd,
+                                                      vv.ddict, vv.ndl, p, q, ctl, part, xcd);
+    return;
+  }
   if (vv.vi) {
     k_spmv_pq_vi<kBlock, kSpmvTile><<<g, kBlock, 0, stream()>>>(A.nblocks, rb, ia, ja, vv.vi,
                                                                 vv.dict, vv.nd, p, q, ctl, part,
@@ -1010,7 +1109,7 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   launch_init(gA, A, ia, ja, a, x, b, r, R0);
   const bool jac = o.kind == P_JACOBI;
   const int jv = jac ? options().jacobi_variant : 0;
-  const ViVals vv = jv == 2 ? ViVals() : vi_prepare(nnz, a);
+  const ViVals vv = jv == 2 ? ViVals() : vi_prepare(nnz, a, n, ia, ja);
   const bool fused = jv == 2, xin_c = jv == 1;
   // fused Jacobi: p alternates between two buffers (p_t is read while
   // p_{t+1} is written); p_1 goes to pp[1]
@@ -1221,7 +1320,7 @@ double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const do
   PcgCtl *ctl = reinterpret_cast<PcgCtl *>(scratch_vec("timed_ctl", sizeof(PcgCtl) / 8 + 1));
   LAP_HIP(hipMemcpyAsync(ctl, &h, sizeof h, hipMemcpyHostToDevice, stream()));
   const unsigned g = spmv_grid(*A);
-  const ViVals vv = vi_prepare(nnz, a);  // the kernel the solve would run
+  const ViVals vv = vi_prepare(nnz, a, n, ia, ja);  // the kernel the solve would run
   auto launch = [&]() { launch_spmv_pq(g, *A, ia, ja, a, p, q, ctl, ws.partials + kR2, vv); };
   launch();
   hipEvent_t e0, e1;
