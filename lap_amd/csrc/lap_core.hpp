@@ -349,9 +349,6 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
 // sums: y = (float)((x - sum a*y) / pivot), pivot in the row.
 void tri_solve32_launch(const TriPlan &p, const int *ja, const float *a, const float *x, float *y,
                         const int *done);
-// FP32 values, FP64 vectors, FP64 row sums.
-void tri_solve32v64_launch(const TriPlan &p, const int *ja, const float *a, const double *x,
-                           double *y, const int *done);
 // Sync-free mode: nonzero once a dependency wait ran out (the result is then
 // invalid); reset by tri_clear_error.
 int tri_error();

@@ -650,14 +650,18 @@ const double *inner_d_r<double>(MixPlan *, const double *d_r, int) {
   return d_r;
 }
 
-// FP32 triangular solve of the inner IC0 step, vectors in VT
-inline void tri_inner(const TriPlan &T, const int *ja, const float *a, const float *x, float *y,
-                      const int *done) {
-  tri_solve32_launch(T, ja, a, x, y, done);
+// The inner IC0 step.  FP32 vectors: the FP32 copy of the factor.  FP64
+// vectors: the FP64 factor itself -- the level-scheduled solves are bound by
+// latency, not bytes, so FP32 factor values would save no time and their
+// rounding costs iterations on badly conditioned systems.
+inline void ic_inner(const IcPlan &P, const float *l32, const float *u32, const float *x, float *y,
+                     float *aux, const int *done) {
+  tri_solve32_launch(*P.lplan, P.lja->as<int>(), l32, x, aux, done);
+  tri_solve32_launch(*P.uplan, P.uja->as<int>(), u32, aux, y, done);
 }
-inline void tri_inner(const TriPlan &T, const int *ja, const float *a, const double *x, double *y,
-                      const int *done) {
-  tri_solve32v64_launch(T, ja, a, x, y, done);
+inline void ic_inner(const IcPlan &P, const float *, const float *, const double *x, double *y,
+                     double *aux, const int *done) {
+  ic_apply_launch(P, x, y, aux, done);
 }
 
 template <class VT>
@@ -704,7 +708,7 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
       return LAP_ERR_PREC;
     }
     auto &slot = M->ic32[icp];
-    if (!slot.first) {
+    if (sizeof(VT) == sizeof(float) && !slot.first) {
       slot.first = dev_alloc(sizeof(float) * (size_t)std::max(icp->nnzL, 1));
       slot.second = dev_alloc(sizeof(float) * (size_t)std::max(icp->nnzL, 1));
       const unsigned g = grid_for((size_t)icp->nnzL, 4, kBlock, 2048);
@@ -713,8 +717,10 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
       k_to_f32<<<g, kBlock, 0, stream()>>>(icp->nnzL, icp->uval->as<double>(),
                                            slot.second->as<float>());
     }
-    l32 = slot.first->as<float>();
-    u32 = slot.second->as<float>();
+    if (slot.first) {
+      l32 = slot.first->as<float>();
+      u32 = slot.second->as<float>();
+    }
   }
 
   auto fvec = [&](const char *tag) {
@@ -778,8 +784,7 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
                                                           (int)gV, Q3, (int)gA);
     } else {
       if (ic) {
-        tri_inner(*icp->lplan, icp->lja->as<int>(), l32, r32, aux32, done);
-        tri_inner(*icp->uplan, icp->uja->as<int>(), u32, aux32, w32, done);
+        ic_inner(*icp, l32, u32, r32, w32, aux32, done);
       } else {
         k_copy_v<VT><<<gV, kBlock, 0, stream()>>>(n, r32, w32, done);
       }

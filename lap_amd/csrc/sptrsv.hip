@@ -54,9 +54,10 @@ namespace {
 // chunk at a time into LDS (two buffers: waves 1.. form the next chunk while
 // lane 0 subtracts the current one in CSR order; the pivot's slot holds
 // +0.0, and x - (+0.0) == x bit for bit), so a long row gets exactly the
-// lane path's arithmetic while its loads proceed in parallel.  AT/VT (value/vector type) = double/double: the FP64 solve;
-// float/float or float/double: FP32 values with FP64 products and sums (the
-// mixed-precision inner solve).
+// lane path's arithmetic while its loads proceed in parallel.  AT/VT
+// (value/vector type) = double/double: the FP64 solve; float/float: FP32
+// values and vectors with FP64 products and sums (the mixed-precision inner
+// solve).
 constexpr int kTriChunk = 2048;
 
 template <class AT, class VT>
@@ -588,10 +589,6 @@ void tri_solve32_launch(const TriPlan &p, const int *ja, const float *a, const f
   tri_mixed_launch<float>(p, ja, a, x, y, done);
 }
 
-void tri_solve32v64_launch(const TriPlan &p, const int *ja, const float *a, const double *x,
-                           double *y, const int *done) {
-  tri_mixed_launch<double>(p, ja, a, x, y, done);
-}
 
 static void tri_entry(int n, int nnz, const int *ia, const int *ja, const double *a,
                       const double *diag, const double *x, double *y, bool lower) {

@@ -81,9 +81,17 @@ def case(name, A, b, kind, tol, maxit=25000, m=1, k=1, full_x=False, binary=True
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also the 256^3 Jacobi case (slow)")
+    ap.add_argument("--big-prec", action="store_true",
+                    help="only the 256^3 GS_std and IC0 cases (tens of minutes)")
     args = ap.parse_args()
     if not oracle.have_ref():
         sys.exit("build oracle/_ref first: bash oracle/build_ref.sh")
+    if args.big_prec:
+        A = grid_laplacian([256, 256, 256], 1e-8)
+        b = rhs_normal(A.n, 12345)
+        for kind in ("GS_std", "ichol"):
+            case(f"c2_{kind}", A, b, kind, 1e-10, binary=False)
+        return
     # 2D 5-point 128x128 + 1e-8 I, b ~ U(-1,1) mean-subtracted, tol 1e-8
     A = grid_laplacian([128, 128], 1e-8)
     b = rhs_uniform(A.n, 12345)

@@ -202,22 +202,31 @@ def _sketch(x, rows=16, seed=777):
     return out, ones
 
 
-@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "c2_line_jacobi.npz")),
-                    reason="256^3 fixture not generated")
-def test_pcg_256cubed_matches_reference():
-    """Full-size configuration: 256^3 7-point Laplacian + 1e-8 I, Jacobi, tol
-    1e-10, against the reference's own serial cg() (1583 iterations).
-    Size-independent checks: iteration count +-1, early residual history,
-    true residual, and the solution through a 16-row random sign sketch and
-    512 sampled entries (mean removed, see module docstring)."""
-    g = dict(np.load(os.path.join(GOLDEN, "c2_line_jacobi.npz"), allow_pickle=False))
+@pytest.fixture(scope="module")
+def c2_system():
     Ah = grid_laplacian([256, 256, 256], 1e-8)
     b = rhs_normal(Ah.n, 12345)
-    A = S.DeviceCSR.from_csr(Ah)
-    P = S.make_preconditioner(A, Ah, "line_jacobi")
+    return Ah, b, S.DeviceCSR.from_csr(Ah)
+
+
+@pytest.mark.parametrize("kind", ["line_jacobi", "GS_std", "ichol"])
+def test_pcg_256cubed_matches_reference(c2_system, kind):
+    """Full-size configuration: 256^3 7-point Laplacian + 1e-8 I, tol 1e-10,
+    Jacobi, GS_std (M = K = 1) and IC0, against the reference's own serial
+    cg() (Jacobi 1583 iterations).  Size-independent checks: iteration count
+    +-1, early residual history, true residual, and the solution through a
+    16-row random sign sketch and 512 sampled entries (mean removed, see
+    module docstring)."""
+    path = os.path.join(GOLDEN, f"c2_{kind}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"256^3 {kind} fixture not generated")
+    g = dict(np.load(path, allow_pickle=False))
+    Ah, b, A = c2_system
+    print(f"[case] 256^3 {kind}", flush=True)
+    P = S.make_preconditioner(A, Ah, kind)
     res = S.pcg(A, S.to_dev(b), P, tol=1e-10, maxit=25000)
     it_ref = int(g["iterations"])
-    print(f"[diag] 256^3: it {res.iterations} vs {it_ref}, {res.gpu_ms:.1f} ms", flush=True)
+    print(f"[diag] 256^3 {kind}: it {res.iterations} vs {it_ref}, {res.gpu_ms:.1f} ms", flush=True)
     assert res.flag == 0 and abs(res.iterations - it_ref) <= 1
     np.testing.assert_allclose(res.history[:10], g["history"][:10], rtol=1e-12)
     assert res.true_res <= 1e-10 * res.res0 * 1.5
@@ -229,7 +238,7 @@ def test_pcg_256cubed_matches_reference():
     sk_c = sk - x.mean() * ones
     ref_c = g["sketch"] - float(g["xmean"]) * ones
     dk = np.linalg.norm(sk_c - ref_c) / np.linalg.norm(ref_c)
-    print(f"[diag] 256^3: sample rel dev {ds:.2e}, sketch rel dev {dk:.2e}", flush=True)
+    print(f"[diag] 256^3 {kind}: sample rel dev {ds:.2e}, sketch rel dev {dk:.2e}", flush=True)
     assert ds <= 1e-6 and dk <= 1e-6
 
 
