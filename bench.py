@@ -334,6 +334,30 @@ def bench_power_law(S, torch, np, tol: float, n: int = 1 << 22, reps: int = 2,
                 log(f"power_law {kind} mixed_vectors {mv}: {m.iterations} iterations "
                     f"{mwall:.3f} s ({wall / mwall:.2f}x)")
             S.set_option("mixed_vectors", 0)
+            if kind == "ichol":
+                # long rows summed as workgroup trees (sptrsv_long 1): no serial
+                # chains, tolerance parity instead of bitwise
+                prev_long = S.set_option("sptrsv_long", 1)
+                try:
+                    res, twall = timed(fp64, 1)
+                    t64 = res[-1]
+                    S.set_option("mixed_vectors", 1)
+                    res, tmwall = timed(lambda: S.pcg_mixed(A, b, P, tol=tol, maxit=25000), 1)
+                    tm = res[-1]
+                finally:
+                    S.set_option("sptrsv_long", prev_long)
+                    S.set_option("mixed_vectors", 0)
+                rec["tree_long_rows"] = {
+                    "fp64": {"s": twall, "iterations": t64.iterations, "flag": t64.flag,
+                             "true_rel_res": t64.true_res / t64.res0,
+                             "rel_diff_vs_sequential_solution": float(
+                                 torch.linalg.norm(t64.x - d.x) / torch.linalg.norm(d.x))},
+                    "mixed_vectors_1": {"s": tmwall, "iterations": tm.iterations,
+                                        "refinements": tm.refinements, "flag": tm.flag,
+                                        "speedup_vs_fp64_tree": twall / tmwall},
+                    "speedup_vs_sequential_fp64": wall / twall}
+                log(f"power_law ichol tree rows: fp64 {t64.iterations} iterations {twall:.3f} s, "
+                    f"mixed {tm.iterations} iterations {tmwall:.3f} s")
             out[kind] = rec
             del P
     finally:

@@ -165,6 +165,7 @@ int lap_set_option(const char *name, int value) {
   else if (!strcmp(name, "fuse_sweeps")) slot = &o.fuse_sweeps;
   else if (!strcmp(name, "spmv_format")) slot = &o.spmv_format;
   else if (!strcmp(name, "mixed_vectors")) slot = &o.mixed_vectors;
+  else if (!strcmp(name, "sptrsv_long")) slot = &o.sptrsv_long;
   else if (!strcmp(name, "sptrsv_mode")) slot = &o.sptrsv_mode;
   else if (!strcmp(name, "spmm_wg_per_cu")) slot = &o.spmm_wg_per_cu;
   else if (!strcmp(name, "sf_wg_per_cu")) slot = &o.sf_wg_per_cu;
@@ -186,6 +187,10 @@ int lap_set_option(const char *name, int value) {
   }
   if (!strcmp(name, "sptrsv_mode") && (value < 0 || value > 2)) {
     set_error(LAP_ERR_ARG, "sptrsv_mode must be 0, 1 or 2");
+    return *slot;
+  }
+  if (!strcmp(name, "sptrsv_long") && (value < 0 || value > 1)) {
+    set_error(LAP_ERR_ARG, "sptrsv_long must be 0 or 1");
     return *slot;
   }
   if (!strcmp(name, "mixed_vectors") && (value < 0 || value > 1)) {

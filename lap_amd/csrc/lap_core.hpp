@@ -233,6 +233,8 @@ struct Options {
   int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
   int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
   int mixed_vectors = 0;   // lap_pcg_mixed inner vectors: 0 FP32, 1 FP64
+  int sptrsv_long = 0;     // level solves' rows > kTriLongRow entries: 0 summed
+                           // in CSR order (bitwise), 1 as a workgroup tree
   int spmv_format = 0;     // PCG SpMV: 0 CSR values, 1 value-indexed when the
                            // matrix has at most 255 distinct values, 2 also
                            // the columns as offset indices (<= 255 distinct

@@ -60,7 +60,10 @@ namespace {
 // solve).
 constexpr int kTriChunk = 2048;
 
-template <class AT, class VT>
+// TREE (option sptrsv_long 1): a long row's products are summed by the
+// workgroup as a fixed-order tree instead -- deterministic, no serial chain,
+// but not the sequential order (tolerance parity, not bitwise).
+template <class AT, class VT, bool TREE = false>
 __global__ __launch_bounds__(kBlock) void k_tri_level(
     int cnt, int nlong, const int4 *__restrict__ desc, const int *__restrict__ ja,
     const AT *__restrict__ a, const double *__restrict__ diag, const VT *x, VT *y,
@@ -92,6 +95,33 @@ __global__ __launch_bounds__(kBlock) void k_tri_level(
   }
   const int lr = (int)blockIdx.x - gshort;
   if (lr >= nlong) return;
+  if (TREE) {
+    __shared__ double red[kBlock / kWave];
+    __shared__ int tpos;
+    const int4 d = desc[cnt + lr];
+    const int row = d.x;
+    if (threadIdx.x == 0) tpos = -1;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll 4
+    for (int e = d.y + (int)threadIdx.x; e < d.z; e += kBlock) {
+      const int c = ja[e];
+      if (c == row)
+        atomicMax(&tpos, e);
+      else
+        s += (double)a[e] * (double)y[c];
+    }
+    const double t = block_sum<kBlock>(s, red);  // ends with a barrier: tpos is final
+    if (threadIdx.x == 0) {
+      double piv = 0.0;
+      if (tpos >= 0)
+        piv = (double)a[tpos];
+      else if (diag)
+        piv = diag[row];
+      y[row] = (VT)(((double)x[row] - t) / piv);
+    }
+    return;
+  }
   __shared__ double prod[2][kTriChunk];
   __shared__ int dpos;
   const int4 d = desc[cnt + lr];
@@ -567,8 +597,12 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
   for (int l = 0; l < p.nlevels; ++l) {
     const int cnt = p.level_long[l] - p.level_ptr[l], nlong = p.level_ptr[l + 1] - p.level_long[l];
     const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock + nlong);
-    k_tri_level<double, double><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l],
-                                                            ja, a, diag, x, y, done);
+    if (options().sptrsv_long == 1)
+      k_tri_level<double, double, true><<<g, kBlock, 0, stream()>>>(
+          cnt, nlong, desc + p.level_ptr[l], ja, a, diag, x, y, done);
+    else
+      k_tri_level<double, double><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l],
+                                                              ja, a, diag, x, y, done);
   }
 }
 
@@ -579,8 +613,12 @@ static void tri_mixed_launch(const TriPlan &p, const int *ja, const float *a, co
   for (int l = 0; l < p.nlevels; ++l) {
     const int cnt = p.level_long[l] - p.level_ptr[l], nlong = p.level_ptr[l + 1] - p.level_long[l];
     const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock + nlong);
-    k_tri_level<float, VT><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l], ja, a,
-                                                       nullptr, x, y, done);
+    if (options().sptrsv_long == 1)
+      k_tri_level<float, VT, true><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l],
+                                                               ja, a, nullptr, x, y, done);
+    else
+      k_tri_level<float, VT><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l], ja,
+                                                         a, nullptr, x, y, done);
   }
 }
 

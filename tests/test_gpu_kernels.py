@@ -308,3 +308,41 @@ def test_trisolve_long_rows_one_launch_bitwise(orc, sync_free):
     A = LONG_ROWS["power_law"]()
     print(f"[case] long rows one-launch n={A.n}", flush=True)
     test_trisolve_bitwise(orc, A)
+
+
+@pytest.fixture
+def tree_long_rows():
+    prev = S.set_option("sptrsv_long", 1)
+    yield
+    S.set_option("sptrsv_long", prev)
+
+
+@pytest.mark.parametrize("name", sorted(LONG_ROWS))
+def test_trisolve_long_rows_tree_tolerance(orc, name, tree_long_rows):
+    """Option sptrsv_long 1 sums a long row's products as a workgroup tree:
+    not the sequential order, so compared within a tolerance -- each solve
+    within 1e-12 of the sequential solve relative to the solution's size,
+    short rows untouched; IC0 PCG on the power-law graph within one iteration
+    and 1e-10 of the oracle's solution."""
+    A = LONG_ROWS[name]()
+    print(f"[case] tree long rows {name} n={A.n}", flush=True)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(A.n)
+    Ls, d, Us = split(A)
+    for T, lower, ref in ((augment_lower(A), True, orc.lower_solve),
+                          (augment_upper(A), False, orc.upper_solve)):
+        got = host(S.tri_solve(S.DeviceCSR.from_csr(T), dev(x), lower))
+        want = ref(T, x)
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    if name == "power_law":
+        b = rng.standard_normal(A.n)
+        b -= b.mean()
+        D = S.DeviceCSR.from_csr(A)
+        P = S.make_preconditioner(D, A, "ichol")
+        res = S.pcg(D, dev(b), P, tol=1e-10, maxit=2000)
+        xo, ito, _, _ = orc.cg(A, b, "ichol", tol=1e-10, maxit=2000)
+        err = np.linalg.norm(host(res.x) - xo) / np.linalg.norm(xo)
+        print(f"[diag] tree IC0 PCG: it {res.iterations} vs oracle {ito}, err {err:.2e}",
+              flush=True)
+        assert res.flag == 0 and abs(res.iterations - ito) <= 1
+        assert err <= 1e-10
