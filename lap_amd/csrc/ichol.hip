@@ -36,15 +36,11 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int cnt, const int *__restri
   if (t < cnt) dst[pos[t]] = src[t];
 }
 
+// One row j of the factor by one lane (entries ascending; each merges row
+// j's earlier entries with row c's off-diagonal entries).
 // bad[0]: count of non-positive / non-finite pivots, bad[1]: first such row
-__global__ __launch_bounds__(kBlock) void k_ic0_level(int cnt, const int *__restrict__ rows,
-                                                      const int *__restrict__ lia,
-                                                      const int *__restrict__ lja,
-                                                      double *__restrict__ lval,
-                                                      int *__restrict__ bad) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= cnt) return;
-  const int j = rows[t];
+__device__ void ic0_row(int j, const int *__restrict__ lia, const int *__restrict__ lja,
+                        double *__restrict__ lval, int *__restrict__ bad) {
   const int p0 = lia[j], p1 = lia[j + 1];
   for (int p = p0; p < p1; ++p) {
     const int c = lja[p];
@@ -72,6 +68,94 @@ __global__ __launch_bounds__(kBlock) void k_ic0_level(int cnt, const int *__rest
     } else {
       lval[p] = acc / lval[lia[c + 1] - 1];
     }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ic0_level(int cnt, const int *__restrict__ rows,
+                                                      const int *__restrict__ lia,
+                                                      const int *__restrict__ lja,
+                                                      double *__restrict__ lval,
+                                                      int *__restrict__ bad) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= cnt) return;
+  ic0_row(rows[t], lia, lja, lval, bad);
+}
+
+// Rows of more than kTriLongRow entries: a workgroup per row.  Row j's
+// columns and values sit in LDS; its entries are taken in order, and for
+// entry (j, c) every thread takes entries of row c (off-diagonal), finds
+// their column among row j's earlier entries by binary search and writes
+// the product L(c,i) * L(j,i) -- or +0.0 where row j has no such column --
+// into LDS at the entry's position; lane 0 then subtracts them in row c's
+// (ascending) order.  The matched products come in the merge's order and
+// x - (+0.0) == x bit for bit, so every entry is exactly ic0_row's.  Rows
+// longer than kIcRow entries are left to one lane (ic0_row).
+constexpr int kIcRow = 2048;
+
+__global__ __launch_bounds__(kBlock) void k_ic0_long(const int *__restrict__ rows,
+                                                     const int *__restrict__ lia,
+                                                     const int *__restrict__ lja,
+                                                     double *__restrict__ lval,
+                                                     int *__restrict__ bad) {
+  __shared__ int cj[kIcRow];
+  __shared__ double vj[kIcRow];
+  __shared__ double prod[kIcRow];
+  const int j = __builtin_amdgcn_readfirstlane(rows[blockIdx.x]);
+  const int p0 = __builtin_amdgcn_readfirstlane(lia[j]);
+  const int len = __builtin_amdgcn_readfirstlane(lia[j + 1]) - p0;
+  if (len > kIcRow) {
+    if (threadIdx.x == 0) ic0_row(j, lia, lja, lval, bad);
+    return;
+  }
+  for (int k = threadIdx.x; k < len; k += kBlock) {
+    cj[k] = lja[p0 + k];
+    vj[k] = lval[p0 + k];  // A's values; each is replaced once computed
+  }
+  __syncthreads();
+  for (int pk = 0; pk < len; ++pk) {
+    const int c = __builtin_amdgcn_readfirstlane(cj[pk]);
+    double acc = vj[pk];
+    if (c == j) {  // the diagonal: minus the squares of the row's entries
+      for (int k = threadIdx.x; k < pk; k += kBlock) prod[k] = vj[k] * vj[k];
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        acc = lds_seq_sum<true>(acc, prod, 0, pk);
+        if (!(acc > 0.0) || !isfinite(acc)) {
+          if (atomicAdd(&bad[0], 1) == 0) bad[1] = j;
+        }
+        const double v = __builtin_sqrt(acc);
+        vj[pk] = v;
+        lval[p0 + pk] = v;
+      }
+    } else {
+      const int s0 = __builtin_amdgcn_readfirstlane(lia[c]);
+      const int lenc = __builtin_amdgcn_readfirstlane(lia[c + 1]) - 1 - s0;
+      for (int base = 0; base < lenc; base += kIcRow) {
+        const int m = min(kIcRow, lenc - base);
+        for (int k = threadIdx.x; k < m; k += kBlock) {
+          const int s = s0 + base + k;
+          const int col = lja[s];
+          int lo = 0, hi = pk;  // row j's entries before (j, c), ascending
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (cj[mid] < col)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+          prod[k] = (lo < pk && cj[lo] == col) ? lval[s] * vj[lo] : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) acc = lds_seq_sum<true>(acc, prod, 0, m);
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        const double v = acc / lval[s0 + lenc];
+        vj[pk] = v;
+        lval[p0 + pk] = v;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -183,10 +267,17 @@ const IcPlan *ic_factor(int n, int nnzA, const int *ia, const int *ja, double *a
   if (nnzL) k_gather<<<gL, kBlock, 0, stream()>>>(nnzL, d_lpos->as<int>(), a, P.lval->as<double>());
   const int *order = P.lplan->order->as<int>();
   for (int l = 0; l < P.lplan->nlevels; ++l) {
-    const int cnt = P.lplan->level_ptr[l + 1] - P.lplan->level_ptr[l];
-    k_ic0_level<<<(cnt + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(
-        cnt, order + P.lplan->level_ptr[l], P.lia->as<int>(), P.lja->as<int>(),
-        P.lval->as<double>(), bad->as<int>());
+    // the level's short rows (one lane each), then its long rows (one
+    // workgroup each); rows of one level never depend on each other
+    const int l0 = P.lplan->level_ptr[l], ll = P.lplan->level_long[l];
+    const int cnt = ll - l0, nlong = P.lplan->level_ptr[l + 1] - ll;
+    if (cnt > 0)
+      k_ic0_level<<<(cnt + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(
+          cnt, order + l0, P.lia->as<int>(), P.lja->as<int>(), P.lval->as<double>(),
+          bad->as<int>());
+    if (nlong > 0)
+      k_ic0_long<<<nlong, kBlock, 0, stream()>>>(order + ll, P.lia->as<int>(), P.lja->as<int>(),
+                                                 P.lval->as<double>(), bad->as<int>());
   }
   if (nnzL) {
     k_scatter<<<gL, kBlock, 0, stream()>>>(nnzL, d_lpos->as<int>(), P.lval->as<double>(), a);
