@@ -228,7 +228,10 @@ def test_pcg_256cubed_matches_reference(c2_system, kind):
     it_ref = int(g["iterations"])
     print(f"[diag] 256^3 {kind}: it {res.iterations} vs {it_ref}, {res.gpu_ms:.1f} ms", flush=True)
     assert res.flag == 0 and abs(res.iterations - it_ref) <= 1
-    np.testing.assert_allclose(res.history[:10], g["history"][:10], rtol=1e-12)
+    # the GPU's dot products sum in another order than the reference's loops:
+    # the early residual norms agree to ~1e-12 (GS_std 1.2e-12 after its
+    # sweeps); a wrong term would be off by orders of magnitude
+    np.testing.assert_allclose(res.history[:10], g["history"][:10], rtol=1e-11)
     assert res.true_res <= 1e-10 * res.res0 * 1.5
     x = res.x.cpu().numpy()
     xs = x[:: max(1, Ah.n // 512)]
