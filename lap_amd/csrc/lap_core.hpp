@@ -122,13 +122,15 @@ __device__ __forceinline__ double lds_seq_sum(double s, const double *p, int k0,
 #pragma unroll
     for (int j = 0; j < 8; ++j) s = SUB ? s - v[j] : s + v[j];
   }
-  const int m = k1 - k;  // 0..7 left
-  double v[7];
+  if (k + 4 <= k1) {
+    double v[4];
 #pragma unroll
-  for (int j = 0; j < 7; ++j) v[j] = j < m ? p[k + j] : 0.0;
+    for (int j = 0; j < 4; ++j) v[j] = p[k + j];
 #pragma unroll
-  for (int j = 0; j < 7; ++j)
-    if (j < m) s = SUB ? s - v[j] : s + v[j];
+    for (int j = 0; j < 4; ++j) s = SUB ? s - v[j] : s + v[j];
+    k += 4;
+  }
+  for (; k < k1; ++k) s = SUB ? s - p[k] : s + p[k];  // 0..3 left: short rows
   return s;
 }
 
