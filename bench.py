@@ -261,8 +261,15 @@ def bench_energy(S, torch, A, b, P, tol: float, solves: int = 3) -> dict:
             torch.cuda.synchronize()
         out[name] = em.report(solves)
     if "fp64" in out and "mixed" in out and "energy_j_per_solve" in out["mixed"]:
-        out["mixed_energy_saving"] = (out["fp64"]["energy_j_per_solve"]
-                                      / out["mixed"]["energy_j_per_solve"])
+        f, m = out["fp64"], out["mixed"]
+        out["mixed_energy_saving"] = f["energy_j_per_solve"] / m["energy_j_per_solve"]
+        if "energy_j_per_solve_time_x_power" in f and "energy_j_per_solve_time_x_power" in m:
+            out["mixed_energy_saving_time_x_power"] = (f["energy_j_per_solve_time_x_power"]
+                                                      / m["energy_j_per_solve_time_x_power"])
+            # the counter-based saving stands only when both readings agree
+            # with their sampled power
+            out["counter_readings_consistent"] = bool(f.get("sources_agree")
+                                                      and m.get("sources_agree"))
     return out
 
 

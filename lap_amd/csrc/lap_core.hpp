@@ -223,7 +223,7 @@ __device__ __forceinline__ BlockWalk block_walk(int nblk, int mode) {
 struct Options {
   int spmv_rows = 256;   // rows per SpMV block (<= kBlock * 2)
   int spmv_walk = 0;     // row-block walk of the persistent SpMV grid (0..5)
-  int vec_grid = 1024;   // workgroups of the streaming vector kernels
+  int vec_grid = 768;    // workgroups of the streaming vector kernels (3 per CU)
   int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;

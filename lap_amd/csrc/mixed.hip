@@ -735,7 +735,10 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   double *Q0 = ws.partials + kQ0, *Q1 = ws.partials + kQ1, *Q2 = ws.partials + kQ2,
          *Q3 = ws.partials + kQ3;
   const unsigned gA = spmv_grid(*A64), gO = spmv_grid(*M->off);
-  const unsigned gV = grid_for((size_t)n, 4, kBlock, (unsigned)options().vec_grid);
+  // FP32 vectors carry half the bytes per element: their kernels run best with
+  // twice the FP64 kernels' workgroups (measured at 256^3: 1536 against 768)
+  const unsigned gV = grid_for((size_t)n, 4, kBlock,
+                               (unsigned)options().vec_grid * (sizeof(VT) == 4 ? 2u : 1u));
   const int walk = options().spmv_walk;
 
   MixCtl h0{};

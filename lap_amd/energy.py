@@ -124,4 +124,10 @@ class EnergyMeter:
         if self.mean_watts is not None:
             out["mean_power_w_sampled"] = self.mean_watts
             out["energy_j_per_solve_time_x_power"] = self.mean_watts * self.seconds / solves
+            # the two sources should tell the same story; a counter that implies
+            # a mean power far from the sampled one (or above the board's limit)
+            # is flagged rather than trusted
+            counter_w = out["mean_power_w_counter"]
+            out["sources_agree"] = bool(abs(counter_w - self.mean_watts)
+                                        <= 0.25 * max(self.mean_watts, 1.0))
         return out
