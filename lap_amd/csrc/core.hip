@@ -168,6 +168,7 @@ int lap_set_option(const char *name, int value) {
   else if (!strcmp(name, "sptrsv_long")) slot = &o.sptrsv_long;
   else if (!strcmp(name, "sptrsv_mode")) slot = &o.sptrsv_mode;
   else if (!strcmp(name, "spmm_wg_per_cu")) slot = &o.spmm_wg_per_cu;
+  else if (!strcmp(name, "spmv_wg_per_cu")) slot = &o.spmv_wg_per_cu;
   else if (!strcmp(name, "sf_wg_per_cu")) slot = &o.sf_wg_per_cu;
   else if (!strcmp(name, "sf_naps")) slot = &o.sf_naps;
   if (!slot) return -1;
@@ -179,6 +180,10 @@ int lap_set_option(const char *name, int value) {
   if ((!strcmp(name, "sf_wg_per_cu") && (value < 1 || value > 16)) ||
       (!strcmp(name, "sf_naps") && (value < 1 || value > 64))) {
     set_error(LAP_ERR_ARG, "option %s: value %d out of range", name, value);
+    return *slot;
+  }
+  if (!strcmp(name, "spmv_wg_per_cu") && (value < 1 || value > 16)) {
+    set_error(LAP_ERR_ARG, "spmv_wg_per_cu must be in [1, 16]");
     return *slot;
   }
   if (!strcmp(name, "spmm_wg_per_cu") && (value < 1 || value > 16)) {

@@ -224,6 +224,7 @@ struct Options {
   int spmv_rows = 256;   // rows per SpMV block (<= kBlock * 2)
   int spmv_walk = 0;     // row-block walk of the persistent SpMV grid (0..5)
   int vec_grid = 768;    // workgroups of the streaming vector kernels (3 per CU)
+  int spmv_wg_per_cu = 8;  // persistent SpMV grid: workgroups per CU
   int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;

@@ -114,7 +114,7 @@ const SpmvPlan *spmv_plan(int n, int nnz, const int *ia, const int *ja) {
 }
 
 unsigned spmv_grid(const SpmvPlan &p) {
-  const int cap = 8 * device_cus();
+  const int cap = options().spmv_wg_per_cu * device_cus();
   return (unsigned)std::max(1, std::min(p.nblocks, cap));
 }
 

@@ -21,6 +21,7 @@ OPTIONS = {
     "sptrsv_mode": ([0, 1, 2], 3, 0),
     "sptrsv_long": ([0, 1], -1, 0),
     "spmm_wg_per_cu": ([1, 16], 17, None),
+    "spmv_wg_per_cu": ([1, 16], 0, 8),
     "sf_wg_per_cu": ([1, 16], 0, None),
     "sf_naps": ([1, 64], 65, None),
 }
