@@ -176,3 +176,43 @@ def test_value_indexed_mixed_precision_bitwise(kind, fmt, vectors):
         S.set_option("mixed_vectors", prev)
     assert (m0.iterations, m0.refinements, m0.flag) == (m1.iterations, m1.refinements, m1.flag)
     assert np.array_equal(bits(m0.x), bits(m1.x))
+
+
+def spmv_both(M, fmt):
+    A = S.DeviceCSR.from_csr(M)
+    x = torch.randn(M.n, dtype=torch.float64, device="cuda")
+    y0, y1 = torch.empty_like(x), torch.empty_like(x)
+    try:
+        S.set_option("spmv_format", 0)
+        S.spmv_timed(A, x, y0, 2, fused=True)
+        S.set_option("spmv_format", fmt)
+        S.spmv_timed(A, x, y1, 2, fused=True)
+    finally:
+        S.set_option("spmv_format", 0)
+        A.forget()
+    return y0, y1
+
+
+@pytest.mark.parametrize("longest", [64, 65])
+def test_delta_indexed_edge_rows(longest):
+    """Empty rows, and one row of exactly 64 entries (encoded) or 65 (the
+    form falls back): the SpMV is bitwise the CSR one either way."""
+    G = grid_laplacian([40, 40], 1e-8)
+    n = G.n
+    rows = [G.ja[G.ia[i]:G.ia[i + 1]] for i in range(n)]
+    vals = [G.a[G.ia[i]:G.ia[i + 1]] for i in range(n)]
+    for i in (3, 500, n - 1):  # empty rows
+        rows[i] = rows[i][:0]
+        vals[i] = vals[i][:0]
+    r = 800  # a long row: its stencil plus columns at distinct offsets
+    extra = np.setdiff1d(np.arange(r - 2 * longest, r + 2 * longest, 3), rows[r])
+    cols = np.sort(np.concatenate([rows[r], extra[:longest - rows[r].shape[0]]]))
+    rows[r] = cols.astype(np.int32)
+    vals[r] = -np.ones(cols.shape[0]) / (1 + np.arange(cols.shape[0]) % 3)
+    ia = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum([c.shape[0] for c in rows], out=ia[1:])
+    from lap_amd.problems import CSR
+    M = CSR(n, ia, np.concatenate(rows).astype(np.int32), np.concatenate(vals))
+    assert np.diff(M.ia).max() == longest
+    y0, y1 = spmv_both(M, 2)
+    assert torch.equal(y0, y1)
