@@ -83,9 +83,17 @@ def main():
     ap.add_argument("--big", action="store_true", help="also the 256^3 Jacobi case (slow)")
     ap.add_argument("--big-prec", action="store_true",
                     help="only the 256^3 GS_std and IC0 cases (tens of minutes)")
+    ap.add_argument("--power-law", action="store_true",
+                    help="only the 2^22 power-law (Chung-Lu) Jacobi case (minutes)")
     args = ap.parse_args()
     if not oracle.have_ref():
         sys.exit("build oracle/_ref first: bash oracle/build_ref.sh")
+    if args.power_law:
+        from lap_amd.problems import chung_lu
+        A = chung_lu(1 << 22)
+        b = rhs_normal(A.n, 12345)
+        case("c3_line_jacobi", A, b, "line_jacobi", 1e-10, binary=False)
+        return
     if args.big_prec:
         A = grid_laplacian([256, 256, 256], 1e-8)
         b = rhs_normal(A.n, 12345)
