@@ -275,6 +275,21 @@ def test_pcg_power_law_matches_reference():
     dk = np.linalg.norm(sk - g["sketch"]) / np.linalg.norm(g["sketch"])
     print(f"[diag] power law 2^22: sample rel dev {ds:.2e}, sketch rel dev {dk:.2e}", flush=True)
     assert ds <= 1e-10 and dk <= 1e-10
+    # the mixed-precision solves of the same system against the reference's
+    # FP64 solution: within 1e3 * tol, the mixed solver's bar
+    for vectors in (1, 0):
+        prev = S.set_option("mixed_vectors", vectors)
+        try:
+            m = S.pcg_mixed(A, S.to_dev(b), P, tol=1e-10, maxit=25000)
+        finally:
+            S.set_option("mixed_vectors", prev)
+        xm = m.x.cpu().numpy()
+        dm = np.linalg.norm(xm[:: max(1, Ah.n // 512)] - g["x_sample"]) / np.linalg.norm(
+            g["x_sample"])
+        print(f"[diag] power law 2^22 mixed (vectors {vectors}): it {m.iterations}, sample rel "
+              f"dev {dm:.2e}", flush=True)
+        assert m.flag == 0 and m.true_res <= 1e-10 * m.res0 * 1.01
+        assert dm <= 1e3 * 1e-10
     A.forget()
 
 
