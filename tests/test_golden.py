@@ -23,14 +23,22 @@ def problem_for(name):
     if name.startswith("c2_"):
         A = grid_laplacian([256, 256, 256], 1e-8)
         return A, rhs_normal(A.n, 12345)
+    if name.startswith("c3_"):
+        from lap_amd.problems import chung_lu
+        A = chung_lu(1 << 22)
+        return A, rhs_normal(A.n, 12345)
     raise KeyError(name)
+
+
+def full_size(path):
+    return os.path.basename(path).startswith(("c2_", "c3_"))
 
 
 def load(path):
     return dict(np.load(path, allow_pickle=False))
 
 
-@pytest.mark.parametrize("path", [f for f in FILES if not os.path.basename(f).startswith("c2_")],
+@pytest.mark.parametrize("path", [f for f in FILES if not full_size(f)],
                          ids=lambda p: os.path.basename(p)[:-4])
 def test_oracle_reproduces_golden(orc, path):
     import hashlib
@@ -51,3 +59,18 @@ def test_oracle_reproduces_golden(orc, path):
     # the reference binary (MM file route) agreed with the library route
     assert int(g["binary_iterations"]) == int(g["iterations"])
     assert abs(float(g["binary_res"]) - g["history"][-1]) <= 1e-15 * g["history"][-1]
+
+
+@pytest.mark.parametrize("path", [f for f in FILES if full_size(f)],
+                         ids=lambda p: os.path.basename(p)[:-4])
+def test_full_size_fixture_inputs(path):
+    """The full-size fixtures (256^3 and the 2^22 power-law system) are too
+    large for the oracle here; their GPU tests regenerate the inputs, so the
+    generators must still produce exactly the inputs the reference ran on."""
+    import hashlib
+    g = load(path)
+    A, b = problem_for(os.path.basename(path)[:-4])
+    h = hashlib.sha256()
+    for arr in (A.ia, A.ja, A.a, b):
+        h.update(np.ascontiguousarray(arr).tobytes())
+    assert h.hexdigest() == str(g["input_sha256"]), "generator drifted from the fixture inputs"
