@@ -114,6 +114,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(const int *in, long long 
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = (int)sums[nb];
 }
 
+}  // namespace
+
 // out[0..n] = exclusive scan of in[0..n); returns the total (or -1 after a
 // HIP error).  ws needs scan_ws_bytes(n).
 size_t scan_ws_bytes(long long n) {
@@ -130,6 +132,8 @@ long long scan_exclusive(const int *in, long long n, int *out, long long *ws) {
   sync();
   return total;
 }
+
+namespace {
 
 __global__ void k_fill_int(long long n, int v, int *dst) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;

@@ -190,8 +190,8 @@ int lap_set_option(const char *name, int value) {
     set_error(LAP_ERR_ARG, "spmm_wg_per_cu must be in [1, 16]");
     return *slot;
   }
-  if (!strcmp(name, "sptrsv_mode") && (value < 0 || value > 2)) {
-    set_error(LAP_ERR_ARG, "sptrsv_mode must be 0, 1 or 2");
+  if (!strcmp(name, "sptrsv_mode") && (value < 0 || value > 3)) {
+    set_error(LAP_ERR_ARG, "sptrsv_mode must be 0, 1, 2 or 3");
     return *slot;
   }
   if (!strcmp(name, "sptrsv_long") && (value < 0 || value > 1)) {

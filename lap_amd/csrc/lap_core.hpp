@@ -229,9 +229,10 @@ struct Options {
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;
   int spmm_wg_per_cu = 4;  // block CG SpMM grid: workgroups per CU
-  int sptrsv_mode = 0;     // triangular solves: 0 one launch per level,
+  int sptrsv_mode = 3;     // triangular solves: 0 one launch per level,
                            // 1 one sync-free launch, 2 one launch with a
-                           // grid barrier per level
+                           // grid barrier per level, 3 one persistent launch
+                           // over chunks of rows with point-to-point waits
   int sf_wg_per_cu = 8;    // sync-free solve: workgroups per CU
   int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
   int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
@@ -254,6 +255,11 @@ inline unsigned grid_for(size_t n, int per_thread = 4, int bs = kBlock, unsigned
   if (cap && g > cap) g = cap;
   return (unsigned)g;
 }
+
+// out[0..n] = exclusive scan of in[0..n) on the device (out[n] = total);
+// returns the total, or -1 after a HIP error.  ws: scan_ws_bytes(n) bytes.
+size_t scan_ws_bytes(long long n);
+long long scan_exclusive(const int *in, long long n, int *out, long long *ws);
 
 // Drops every cached plan (SpMV, triangular, IC0, mixed, batch) whose key
 // holds the device address p.
@@ -329,6 +335,24 @@ struct TriPlan {
   bool has_diag = true;        // every row holds its pivot
   int nchunks = 0;             // sync-free mode: <= 64 rows of one level each
   DevBufPtr chunk_ptr;         // int[nchunks + 1], offsets into order
+  // Chunked schedule (one persistent workgroup per chunk of consecutive rows
+  // in solve order; built when every row holds at most 8 entries).  Chunk c
+  // runs its rows level by level ("steps"); before a step it waits until
+  // each earlier chunk it depends on has published the level it needs.
+  struct Chunks {
+    int nchunks = 0, nsteps = 0, nwaits = 0;
+    DevBufPtr cstep;  // int[nchunks + 1]: first step of each chunk
+    DevBufPtr sptr;   // int[nsteps + 1]: first position of each step
+    DevBufPtr slev;   // int[nsteps]: the step's level
+    DevBufPtr wptr;   // int[nsteps + 1]: first wait of each step
+    DevBufPtr wchk;   // int[nwaits]: chunk waited on
+    DevBufPtr wlev;   // int[nwaits]: level it must have published
+    DevBufPtr crow;   // int[n]: rows in (chunk, level, row) order
+    DevBufPtr cmap;   // int[n]: level-order position (into desc) of crow[t]
+    DevBufPtr code;   // int[n * w]: each record's dependencies (see sptrsv.hip)
+    int w = 0;        // record width
+    long long cost = 0;  // simulated pipeline length (steps)
+  } ch;
 };
 // Rows longer than this many entries (pivot included) are solved by a
 // workgroup each in the level launches.
@@ -340,6 +364,8 @@ const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower
 struct TriEll {
   int w = 0;
   DevBufPtr ja, a;
+  DevBufPtr crec;  // the chunked schedule's records for these values
+  DevBufPtr xc;    // a solve's right-hand side in that schedule's order
 };
 // Packs (ja, a) of the plan's matrix when its rows hold at most 8 entries.
 void tri_ell_build(const TriPlan &p, const int *ja, const double *a, TriEll &e);
@@ -350,6 +376,9 @@ void tri_ell_build(const TriPlan &p, const int *ja, const double *a, TriEll &e);
 void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
                       const double *diag, const double *x, double *y, const int *ctl_done,
                       const TriEll *ell = nullptr);
+// The number of kernel launches tri_solve_launch makes for these arguments
+// (x == y: in place).
+int tri_launches(const TriPlan &p, const TriEll *ell, bool in_place);
 // The same level-scheduled solve over FP32 values and vectors with FP64 row
 // sums: y = (float)((x - sum a*y) / pivot), pivot in the row.
 void tri_solve32_launch(const TriPlan &p, const int *ja, const float *a, const float *x, float *y,

@@ -511,113 +511,153 @@ __global__ __launch_bounds__(kBlock) void k_copy_v(int n, const VT *__restrict__
 
 // ---------------------------------------------------------------- plans
 
+// Off-diagonal pattern of A: entries per row without the diagonal.
+__global__ __launch_bounds__(kBlock) void k_mx_count(int n, const int *__restrict__ ia,
+                                                     const int *__restrict__ ja,
+                                                     int *__restrict__ cnt) {
+  for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
+    int c = 0;
+    for (int e = ia[row]; e < ia[row + 1]; ++e) c += ja[e] != row;
+    cnt[row] = c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mx_pattern(int n, const int *__restrict__ ia,
+                                                       const int *__restrict__ ja,
+                                                       const int *__restrict__ io,
+                                                       int *__restrict__ jo) {
+  for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
+    int pos = io[row];
+    for (int e = ia[row]; e < ia[row + 1]; ++e)
+      if (ja[e] != row) jo[pos++] = ja[e];
+  }
+}
+
+// The inner operator's values from A's, every solve: FP32 off-diagonals (in
+// the off-diagonal pattern's order; a64 gets the same entries in FP64 when
+// not null) and the FP64 diagonal plus what rounding took from the row's
+// off-diagonals, accumulated in CSR order.
+__global__ __launch_bounds__(kBlock) void k_mx_values(int n, const int *__restrict__ ia,
+                                                      const int *__restrict__ ja,
+                                                      const double *__restrict__ a,
+                                                      const int *__restrict__ io,
+                                                      float *__restrict__ ao,
+                                                      double *__restrict__ dg,
+                                                      double *__restrict__ a64) {
+  for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
+    double d = 0.0, comp = 0.0;
+    int pos = io[row];
+    for (int e = ia[row]; e < ia[row + 1]; ++e) {
+      const double v = a[e];
+      if (ja[e] == row) {
+        d += v;
+      } else {
+        const float f = (float)v;
+        ao[pos] = f;
+        if (a64) a64[pos] = v;
+        ++pos;
+        comp += v - (double)f;
+      }
+    }
+    dg[row] = d + comp;
+  }
+}
+
+// Per sparsity pattern (ia, ja, n, nnz): the off-diagonal pattern, built on
+// the device once.  Everything derived from values (the FP32 off-diagonals,
+// the compensated diagonal, the value- and delta-indexed copies, the FP32
+// 1/diag and IC0 factor) is rebuilt on the device at every solve, since the
+// caller may change A, d_r or the factor in place between solves.
 struct MixPlan {
   int n = 0, nnz_off = 0;
-  DevBufPtr ia_o, ja_o, a32_o, diag;
-  // value-indexed copy of a32_o (when it has at most 255 distinct values):
-  // one byte per off-diagonal entry and the table of the values
-  DevBufPtr vi_o, dict_o;
-  int nd = 0;
-  // delta-indexed copy (when also the column offsets col - row take at most
-  // 255 values and rows hold at most 64 entries): code = value index |
-  // offset index << 8
-  DevBufPtr code_o, ddict_o;
-  int ndl = 0;
+  DevBufPtr ia_o, ja_o;  // off-diagonal pattern
+  DevBufPtr a32_o, diag; // this solve's values
+  // value-indexed copy (spmv_format >= 1, at most 255 distinct off-diagonal
+  // values): one byte per entry and the FP32 table; the delta-indexed copy
+  // (spmv_format 2, also at most 255 distinct col - row and rows of at most
+  // 64 entries): code = value index | offset index << 8
+  DevBufPtr a64_o, vi_o, dict64, dict_o, code_o, ddict_o;
+  int nd = 0, ndl = 0;
+  DevBufPtr dr32, l32, u32;  // this solve's FP32 1/diag and IC0 factor
   const SpmvPlan *off = nullptr;
-  std::map<const double *, DevBufPtr> dr32;  // by d_r pointer
-  std::map<const IcPlan *, std::pair<DevBufPtr, DevBufPtr>> ic32;
 };
-using MixKey = std::tuple<const int *, const int *, const double *, int, int>;
+using MixKey = std::tuple<const int *, const int *, int, int>;
 std::map<MixKey, std::unique_ptr<MixPlan>> g_mix;
 
-MixPlan *mix_plan(int n, int nnz, const int *ia, const int *ja, const double *a) {
-  MixKey key{ia, ja, a, n, nnz};
+DevBufPtr &sized(DevBufPtr &b, size_t bytes) {
+  if (!b || b->bytes < bytes) b = dev_alloc(bytes);
+  return b;
+}
+
+MixPlan *mix_plan(int n, int nnz, const int *ia, const int *ja) {
+  MixKey key{ia, ja, n, nnz};
   auto it = g_mix.find(key);
   if (it != g_mix.end()) return it->second.get();
-  std::vector<int> h_ia = dev_download(ia, (size_t)n + 1);
-  const int total = h_ia[n];
-  std::vector<int> h_ja = dev_download(ja, (size_t)total);
-  std::vector<double> h_a = dev_download(a, (size_t)total);
-  std::vector<int> io((size_t)n + 1, 0), jo;
-  std::vector<float> ao;
-  std::vector<double> dg((size_t)n, 0.0);
-  jo.reserve((size_t)total);
-  ao.reserve((size_t)total);
-  for (int i = 0; i < n; ++i) {
-    double comp = 0.0;  // what rounding took from the row's off-diagonals
-    for (int e = h_ia[i]; e < h_ia[i + 1]; ++e) {
-      if (h_ja[e] == i) {
-        dg[i] += h_a[e];
-      } else {
-        const float v = (float)h_a[e];
-        jo.push_back(h_ja[e]);
-        ao.push_back(v);
-        comp += h_a[e] - (double)v;
-      }
-    }
-    dg[i] += comp;
-    io[i + 1] = (int)jo.size();
-  }
   auto P = std::make_unique<MixPlan>();
   P->n = n;
-  P->nnz_off = (int)jo.size();
-  P->ia_o = dev_upload(io);
-  P->ja_o = dev_upload(jo);
-  P->a32_o = dev_upload(ao);
-  P->diag = dev_upload(dg);
-  {  // the distinct FP32 values, ascending by bit pattern
-    std::vector<unsigned> bitsv(ao.size());
-    std::memcpy(bitsv.data(), ao.data(), ao.size() * sizeof(float));
-    std::vector<unsigned> keys(bitsv);
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-    if (!keys.empty() && keys.size() <= 255) {
-      std::vector<unsigned char> vi(ao.size());
-      for (size_t e = 0; e < ao.size(); ++e)
-        vi[e] = (unsigned char)(std::lower_bound(keys.begin(), keys.end(), bitsv[e]) -
-                                keys.begin());
-      std::vector<float> dict(keys.size());
-      std::memcpy(dict.data(), keys.data(), keys.size() * sizeof(float));
-      P->vi_o = dev_upload(vi);
-      P->dict_o = dev_upload(dict);
-      P->nd = (int)keys.size();
-      // the offsets
-      std::vector<int> deltas;
-      int longest = 0;
-      for (int i = 0; i < n; ++i) {
-        longest = std::max(longest, io[i + 1] - io[i]);
-        for (int e = io[i]; e < io[i + 1]; ++e) deltas.push_back(jo[e] - i);
-      }
-      std::vector<int> dk(deltas);
-      std::sort(dk.begin(), dk.end());
-      dk.erase(std::unique(dk.begin(), dk.end()), dk.end());
-      if (longest <= 64 && !dk.empty() && dk.size() <= 255) {
-        std::vector<unsigned short> code(ao.size());
-        for (size_t e = 0; e < ao.size(); ++e)
-          code[e] = (unsigned short)(vi[e] | (std::lower_bound(dk.begin(), dk.end(), deltas[e]) -
-                                              dk.begin())
-                                                 << 8);
-        P->code_o = dev_upload(code);
-        P->ddict_o = dev_upload(dk);
-        P->ndl = (int)dk.size();
-      }
-    }
-  }
+  P->ia_o = dev_alloc(sizeof(int) * ((size_t)n + 1));
+  DevBuf cnt(sizeof(int) * ((size_t)n + 1)), ws(scan_ws_bytes(n) + 16);
+  const unsigned g = grid_for((size_t)n, 1, kBlock, 65536);
+  k_mx_count<<<g, kBlock, 0, stream()>>>(n, ia, ja, cnt.as<int>());
+  const long long total = scan_exclusive(cnt.as<int>(), n, P->ia_o->as<int>(),
+                                         ws.as<long long>());
+  if (total < 0) return nullptr;
+  P->nnz_off = (int)total;
+  P->ja_o = dev_alloc(sizeof(int) * (size_t)std::max<long long>(total, 1));
+  k_mx_pattern<<<g, kBlock, 0, stream()>>>(n, ia, ja, P->ia_o->as<int>(), P->ja_o->as<int>());
+  P->a32_o = dev_alloc(sizeof(float) * (size_t)std::max<long long>(total, 1));
+  P->diag = dev_alloc(sizeof(double) * (size_t)n);
+  sync();
   P->off = spmv_plan(n, P->nnz_off, P->ia_o->as<int>(), P->ja_o->as<int>());
-  if (!P->off) return nullptr;
+  if (!P->off || lap_last_status() == LAP_ERR_HIP) return nullptr;
   MixPlan *raw = P.get();
   g_mix[key] = std::move(P);
   return raw;
 }
 
-float *to_f32_cached(std::map<const double *, DevBufPtr> &cache, const double *src, int n) {
-  auto &slot = cache[src];
-  if (!slot) {
-    slot = dev_alloc(sizeof(float) * (size_t)n);
-    k_to_f32<<<grid_for((size_t)n, 4, kBlock, 2048), kBlock, 0, stream()>>>(n, src,
-                                                                            slot->as<float>());
+// This solve's inner-operator values (and indexed copies when `format` asks).
+bool mix_values(MixPlan &M, const int *ia, const int *ja, const double *a, int format) {
+  const int n = M.n;
+  const size_t no = (size_t)std::max(M.nnz_off, 1);
+  double *a64 = nullptr;
+  if (format >= 1) a64 = sized(M.a64_o, sizeof(double) * no)->as<double>();
+  k_mx_values<<<grid_for((size_t)n, 1, kBlock, 65536), kBlock, 0, stream()>>>(
+      n, ia, ja, a, M.ia_o->as<int>(), M.a32_o->as<float>(), M.diag->as<double>(), a64);
+  M.nd = M.ndl = 0;
+  if (format >= 1 && M.nnz_off > 0) {
+    sized(M.dict64, sizeof(double) * 256);
+    sized(M.dict_o, sizeof(float) * 256);
+    int nv = 0;
+    if (format == 2) {
+      sized(M.code_o, sizeof(unsigned short) * no);
+      sized(M.ddict_o, sizeof(int) * 256);
+      int ndl = 0;
+      const int r = dvi_encode(n, M.nnz_off, M.ia_o->as<int>(), M.ja_o->as<int>(), a64, 64,
+                               M.code_o->as<unsigned short>(), M.dict64->as<double>(),
+                               M.ddict_o->as<int>(), &nv, &ndl);
+      if (r < 0) return false;
+      if (r == 1) M.ndl = ndl;
+    }
+    if (M.ndl == 0) {  // format 1, or 2 falling back
+      sized(M.vi_o, no);
+      nv = vi_encode(M.nnz_off, a64, M.vi_o->as<unsigned char>(), M.dict64->as<double>());
+      if (nv < 0) return false;
+    }
+    // the table in FP32: fl32 of each distinct double (fl32 may map two of
+    // them to one float; each index still gives its entry's FP32 value)
+    if (nv > 0)
+      k_to_f32<<<1, kBlock, 0, stream()>>>(nv, M.dict64->as<double>(), M.dict_o->as<float>());
+    M.nd = nv;
   }
-  return slot->as<float>();
+  return lap_last_status() != LAP_ERR_HIP;
+}
+
+float *to_f32(DevBufPtr &buf, const double *src, int n) {
+  sized(buf, sizeof(float) * (size_t)std::max(n, 1));
+  if (n > 0)
+    k_to_f32<<<grid_for((size_t)n, 4, kBlock, 2048), kBlock, 0, stream()>>>(n, src,
+                                                                         buf->as<float>());
+  return buf->as<float>();
 }
 
 }  // namespace
@@ -626,7 +666,7 @@ void release_mix_plans() { g_mix.clear(); }
 void forget_mix_plans(const void *p) {
   for (auto it = g_mix.begin(); it != g_mix.end();) {
     const MixKey &k = it->first;
-    if (std::get<0>(k) == p || std::get<1>(k) == p || std::get<2>(k) == p)
+    if (std::get<0>(k) == p || std::get<1>(k) == p)
       it = g_mix.erase(it);
     else
       ++it;
@@ -643,7 +683,7 @@ template <class VT>
 const VT *inner_d_r(MixPlan *M, const double *d_r, int n);
 template <>
 const float *inner_d_r<float>(MixPlan *M, const double *d_r, int n) {
-  return to_f32_cached(M->dr32, d_r, n);
+  return to_f32(M->dr32, d_r, n);
 }
 template <>
 const double *inner_d_r<double>(MixPlan *, const double *d_r, int) {
@@ -689,8 +729,8 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   }
   const SpmvPlan *A64 = spmv_plan(n, nnz, ia, ja);
   if (!A64) return lap_last_status();
-  MixPlan *M = mix_plan(n, nnz, ia, ja, a);
-  if (!M) return lap_last_status();
+  MixPlan *M = mix_plan(n, nnz, ia, ja);
+  if (!M || !mix_values(*M, ia, ja, a, options().spmv_format)) return lap_last_status();
   const VT *d_r32 = nullptr;
   if (jac) {
     if (!is_device_ptr(pd->d_r)) {
@@ -707,19 +747,9 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
       set_error(LAP_ERR_PREC, "pcg_mixed: no IC0 factor for ichol_vals (call initialize_ichol)");
       return LAP_ERR_PREC;
     }
-    auto &slot = M->ic32[icp];
-    if (sizeof(VT) == sizeof(float) && !slot.first) {
-      slot.first = dev_alloc(sizeof(float) * (size_t)std::max(icp->nnzL, 1));
-      slot.second = dev_alloc(sizeof(float) * (size_t)std::max(icp->nnzL, 1));
-      const unsigned g = grid_for((size_t)icp->nnzL, 4, kBlock, 2048);
-      k_to_f32<<<g, kBlock, 0, stream()>>>(icp->nnzL, icp->lval->as<double>(),
-                                           slot.first->as<float>());
-      k_to_f32<<<g, kBlock, 0, stream()>>>(icp->nnzL, icp->uval->as<double>(),
-                                           slot.second->as<float>());
-    }
-    if (slot.first) {
-      l32 = slot.first->as<float>();
-      u32 = slot.second->as<float>();
+    if (sizeof(VT) == sizeof(float)) {  // this solve's FP32 copy of the factor
+      l32 = to_f32(M->l32, icp->lval->as<double>(), icp->nnzL);
+      u32 = to_f32(M->u32, icp->uval->as<double>(), icp->nnzL);
     }
   }
 
@@ -737,8 +767,16 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   const unsigned gA = spmv_grid(*A64), gO = spmv_grid(*M->off);
   // FP32 vectors carry half the bytes per element: their kernels run best with
   // twice the FP64 kernels' workgroups (measured at 256^3: 1536 against 768)
-  const unsigned gV = grid_for((size_t)n, 4, kBlock,
-                               (unsigned)options().vec_grid * (sizeof(VT) == 4 ? 2u : 1u));
+  // (capped so that the two partials per workgroup of k_mx_update stay
+  // inside their kMaxPartialBlocks region)
+  const unsigned gV = grid_for(
+      (size_t)n, 4, kBlock,
+      std::min<unsigned>((unsigned)options().vec_grid * (sizeof(VT) == 4 ? 2u : 1u),
+                         kMaxPartialBlocks / 2));
+  if (2 * gA > (unsigned)kMaxPartialBlocks || gO > (unsigned)kMaxPartialBlocks) {
+    set_error(LAP_ERR_ARG, "pcg_mixed: SpMV grids %u/%u exceed the partial-sum regions", gA, gO);
+    return LAP_ERR_ARG;
+  }
   const int walk = options().spmv_walk;
 
   MixCtl h0{};

@@ -1160,11 +1160,12 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   // the next launch)
   const int *poll = (fused || xin_c) ? &ctl->pend_done : done;
   int per_iter = fused ? 2 : jac ? 3 : 6;
-  const bool one_launch_solves = options().sptrsv_mode != 0;  // 2 launches per solve
   if (o.kind == P_ICHOL)
-    per_iter += one_launch_solves ? 4 : o.ic->lplan->nlevels + o.ic->uplan->nlevels;
+    per_iter += tri_launches(*o.ic->lplan, &o.ic->lell, false) +
+                tri_launches(*o.ic->uplan, &o.ic->uell, false);
   if (o.kind == P_GS_STD)
-    per_iter += ((one_launch_solves ? 4 : o.Lt->nlevels + o.Ut->nlevels) + 4) * o.k;
+    per_iter += (tri_launches(*o.Lt, &o.lell, false) + tri_launches(*o.Ut, &o.uell, false) + 4) *
+                o.k;
   if (o.kind == P_IT_JACOBI || o.kind == P_GS_IT || o.kind == P_GS_IT2)
     per_iter += 8 * o.m * (o.k + 2);
   int G = std::max(1, std::min(options().graph_iters, 4096 / per_iter));

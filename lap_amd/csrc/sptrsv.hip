@@ -37,7 +37,24 @@
 // workgroup is resident; every wait is bounded and raises the shared error
 // flag when it runs out, so the kernel always ends.  Same row arithmetic as
 // the level launches: bitwise identical.
+//
+// Chunked mode (option sptrsv_mode 3): one persistent launch per solve, one
+// workgroup per chunk of consecutive rows in solve order (all workgroups
+// resident: at most one per CU).  A chunk runs its rows level by level; a
+// level of a chunk is a "step", one lane per row, rows packed 4 or 8 wide.
+// A row's dependencies lie in earlier steps of its own chunk or in earlier
+// chunks, never later ones, so the chunks form a pipeline: before a step,
+// wave 0 waits until every earlier chunk the step depends on has published
+// (in its progress counter) the highest level the step needs from it --
+// point-to-point waits, listed per step by the analysis and pruned to the
+// ones not already implied by an earlier wait of the chunk.  y is stored with
+// agent-scope stores and dependencies are read with agent-scope loads (both
+// coherent across the XCDs' L2s without cache write-backs); a step's stores
+// are complete (s_waitcnt) before its level is published.  Every wait is
+// bounded and raises the shared error flag when it runs out, so the kernel
+// always ends.  Same row arithmetic as the level launches: bitwise identical.
 #include <algorithm>
+#include <cstdlib>
 
 #include "lap_core.hpp"
 
@@ -390,6 +407,308 @@ __global__ __launch_bounds__(BS) void k_tri_barrier(
   }
 }
 
+// ---------------------------------------------------------- chunked mode
+constexpr int kChMinRows = 2048;  // rows per chunk at least
+constexpr int kChMax = 1024;      // chunks at most (the kernel's LDS table)
+constexpr int kChStepRows = kBlock;  // rows per step of a chunk at most (one per lane)
+constexpr int kChRing = 4;        // steps of results kept in LDS
+constexpr int kChSteps = 512;     // steps of a chunk at most (LDS table)
+constexpr int kChWaits = 1024;    // waits of a chunk at most (LDS table)
+constexpr unsigned kChSpinLimit = 1u << 22;  // polls of one wait before the solve fails
+
+__global__ __launch_bounds__(kBlock) void k_ch_reset(int nchunks, int *progress,
+                                                     const int *__restrict__ done) {
+  if (done && *done) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nchunks; i += gridDim.x * kBlock)
+    __hip_atomic_store(&progress[i], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations, not for its outstanding global loads (__syncthreads would wait
+// for every memory operation of the wave).  The signal fences keep the
+// compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// s_waitcnt encoding (gfx9) for vmcnt(N) alone: lgkmcnt and expcnt at their
+// maxima.
+constexpr unsigned vmcnt_only(unsigned n) {
+  return (n & 15u) | ((n >> 4) << 14) | (7u << 4) | (15u << 8);
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void global_void_t;
+
+// The chunked schedule's records, one per row in schedule order (bytes):
+// [0, 4W) the off-diagonal dependency codes in CSR order (-1 padding, >= 0 a
+// row of y, <= -2 the LDS ring slot -(code + 2)); [4W, 4W + 4) the row;
+// [4W + 16, 12W + 16) the off-diagonal values; [12W + 16, 12W + 24) the pivot.
+template <int W>
+struct ChRec {
+  static constexpr int bytes = 12 * W + 32;
+  static constexpr int row = W;                  // int index
+  static constexpr int val = (4 * W + 16) / 8;   // double index
+  static constexpr int piv = (12 * W + 16) / 8;  // double index
+  static constexpr int dma = bytes / 4;          // 16-byte loads of 64 lanes per kBlock records
+};
+
+// One workgroup per chunk: kBlock compute lanes (one row each per step) and
+// one prefetch wave.  progress[c]: highest level of chunk c whose rows are
+// all complete and visible; state[1]: error flag (a wait ran out).
+//  * the prefetch wave copies each step's records and right-hand sides (xc:
+//    x in schedule order, gathered beforehand by k_ch_gather) into an LDS
+//    slot D steps ahead with direct global-to-LDS loads; before each step
+//    ends it waits, by count, for the next step's slot only (its loads are
+//    all of one kind and complete in order).  No compute lane ever waits on
+//    a copy;
+//  * a step starts with compute wave 0 checking its waits against a table
+//    of the levels already seen published, which that wave refreshes by
+//    polling the next step's producers while the current step runs.  Only
+//    when a wait is not covered does the chunk block: it first publishes
+//    everything it has completed, so that no chunk waits on finished but
+//    unpublished work, and polls until the levels it needs are published;
+//  * each row reads rows of this chunk from the last kChRing steps out of
+//    the LDS ring and the others from y (agent-scope loads), and writes its
+//    result to the ring and to y (agent-scope store).  A step ends with the
+//    lanes waiting, by count, for the previous step's store only and a
+//    barrier, after which the previous step's level is published: one step
+//    behind, so that no lane waits on a store it has just issued.
+template <int W, int D, int NS>
+__global__ __launch_bounds__(kBlock + kWave) void k_tri_chunks(
+    const int *__restrict__ cstep, const int *__restrict__ sptr, const int *__restrict__ slev,
+    const int *__restrict__ wptr, const int *__restrict__ wchk, const int *__restrict__ wlev,
+    const char *__restrict__ rec, const double *xc, double *y,
+    int *progress, int *state, const int *__restrict__ done) {
+  if (done && *done) return;
+  using R = ChRec<W>;
+  constexpr int XB = kBlock * 8;  // the step's right-hand sides
+  constexpr int SLOT = kBlock * R::bytes + XB;
+  constexpr int PER = R::dma + XB / (kWave * 4);  // loads per step: records, right-hand sides
+  static_assert((D - 1) * PER <= 63, "vmcnt range");
+  static_assert(D < NS, "a slot in use is never refilled");
+  __shared__ __attribute__((aligned(16))) char slots[NS * SLOT];
+  __shared__ int seen[kChMax];  // highest level seen published, per chunk
+  __shared__ double ring[kChRing * kBlock];
+  __shared__ int tsptr[kChSteps + 1], tslev[kChSteps], twptr[kChSteps + 1];
+  __shared__ int twchk[kChWaits], twlev[kChWaits];
+  __shared__ int failed, must_wait;
+  const int c = blockIdx.x;
+  const bool pf = threadIdx.x >= kBlock;  // the prefetch wave
+  const int p = threadIdx.x;              // a compute lane's row position in a step
+  const int k0 = __builtin_amdgcn_readfirstlane(cstep[c]);
+  const int ns = __builtin_amdgcn_readfirstlane(cstep[c + 1]) - k0;
+  const int q0 = wptr[k0];
+  const int nw = wptr[k0 + ns] - q0;
+  constexpr int nt = kBlock + kWave;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += nt) seen[i] = -1;
+  for (int i = threadIdx.x; i <= ns; i += nt) {
+    tsptr[i] = sptr[k0 + i];
+    twptr[i] = wptr[k0 + i] - q0;
+    if (i < ns) tslev[i] = slev[k0 + i];
+  }
+  for (int i = threadIdx.x; i < nw; i += nt) {
+    twchk[i] = wchk[q0 + i];
+    twlev[i] = wlev[q0 + i];
+  }
+  if (threadIdx.x == 0) {
+    failed = 0;
+    must_wait = 0;
+  }
+  __syncthreads();
+  const int tfirst = tsptr[0];
+  const int lane = (int)threadIdx.x - kBlock;  // prefetch wave lane
+  // prefetch wave: step rel's records into its slot (for rel >= ns a
+  // harmless copy of the chunk's first ones, so every step issues as many)
+  auto copy_records = [&](int rel) {
+    const int t0 = rel < ns ? tsptr[rel] : tfirst;
+    char *slot = slots + (rel % NS) * SLOT;
+    const char *src = rec + (size_t)t0 * R::bytes + lane * 16;
+#pragma unroll
+    for (int i = 0; i < R::dma; ++i)
+      __builtin_amdgcn_global_load_lds((global_void_t *)(src + i * 1024),
+                                       (lds_void_t *)(slot + i * 1024), 16, 0, 0);
+    const char *xs = reinterpret_cast<const char *>(xc + t0) + lane * 4;
+#pragma unroll
+    for (int i = 0; i < XB / (kWave * 4); ++i)
+      __builtin_amdgcn_global_load_lds((global_void_t *)(xs + i * 256),
+                                       (lds_void_t *)(slot + kBlock * R::bytes + i * 256), 4, 0,
+                                       0);
+  };
+  if (pf) {
+#pragma unroll
+    for (int r = 0; r < D; ++r) copy_records(r);
+    __builtin_amdgcn_s_waitcnt(vmcnt_only((D - 1) * PER));  // step 0's slot
+  }
+  for (int rel = 0; rel < ns; ++rel) {
+    if (threadIdx.x < kWave) {  // compute wave 0: are the step's waits already satisfied?
+      bool ready = true;
+      for (int q = twptr[rel] + (int)threadIdx.x; q < twptr[rel + 1]; q += kWave)
+        ready = ready && seen[twchk[q]] >= twlev[q];
+      const bool all = __builtin_amdgcn_ballot_w64(!ready) == 0;
+      if (threadIdx.x == 0) must_wait = !all;
+    }
+    lds_barrier();  // must_wait set; the previous step's ring entries and slot done
+    if (__builtin_amdgcn_readfirstlane(must_wait)) {
+      // about to block: first publish the previous step (its stores complete),
+      // so that no chunk ever waits on work that is done but unpublished
+      if (!pf) __builtin_amdgcn_s_waitcnt(0);
+      lds_barrier();
+      if (threadIdx.x == 0 && rel > 0)
+        __hip_atomic_store(&progress[c], tslev[rel - 1], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      if (threadIdx.x < kWave) {  // wave 0: the step's waits, one lane per producer
+        int ok = 1;
+        const int qe = twptr[rel + 1];
+        for (int q = twptr[rel] + (int)threadIdx.x; q < qe; q += kWave) {
+          const int pc = twchk[q], need = twlev[q];
+          if (seen[pc] >= need) continue;
+          unsigned spins = 0;
+          int v = __hip_atomic_load(&progress[pc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          while (v < need) {
+            if (++spins > kChSpinLimit ||
+                __hip_atomic_load(&state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+              ok = 0;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            v = __hip_atomic_load(&progress[pc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          seen[pc] = v;
+          if (!ok) break;
+        }
+        if (!ok) {
+          __hip_atomic_store(&state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          failed = 1;
+        }
+      }
+      lds_barrier();
+      if (__builtin_amdgcn_readfirstlane(failed)) break;
+    }
+    if (pf) {
+      copy_records(rel + D);
+      // the next step's slot is complete; later ones may still be in flight
+      __builtin_amdgcn_s_waitcnt(vmcnt_only((D - 1) * PER));
+    } else {
+      // wave 0: poll a producer of the next step not yet seen far enough,
+      // issued before this step's loads and read after them
+      int poll_pc = -1, poll_v = 0;
+      if (threadIdx.x < kWave && rel + 1 < ns) {
+        const int q = twptr[rel + 1] + (int)threadIdx.x;
+        if (q < twptr[rel + 2] && seen[twchk[q]] < twlev[q]) {
+          poll_pc = twchk[q];
+          poll_v = __hip_atomic_load(&progress[poll_pc], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      const bool has_row = tsptr[rel] + p < tsptr[rel + 1];
+      int row = 0;
+      double r = 0.0;
+      if (has_row) {
+        const char *slot = slots + (rel % NS) * SLOT;
+        const int *ri = reinterpret_cast<const int *>(slot + p * R::bytes);
+        const double *rd = reinterpret_cast<const double *>(slot + p * R::bytes);
+        int cc[W];
+        double yv[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) cc[j] = ri[j];
+#pragma unroll
+        for (int j = 0; j < W; ++j)  // the dependencies on y first: one wait for all
+          yv[j] = cc[j] >= 0 ? sf_load(y + cc[j]) : 0.0;
+        row = ri[R::row];
+        const double xr = reinterpret_cast<const double *>(slot + kBlock * R::bytes)[p];
+        const double piv = rd[R::piv];
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          if (cc[j] < -1) yv[j] = ring[-(cc[j] + 2)];
+        double acc = xr;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          if (cc[j] == -1) break;
+          acc -= rd[R::val + j] * yv[j];
+        }
+        r = acc / piv;
+        ring[(rel % kChRing) * kBlock + p] = r;
+      }
+      if (poll_pc >= 0) atomicMax(&seen[poll_pc], poll_v);
+      if (has_row) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(y + row),
+                           (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      }
+      // The loads of this step were consumed above, so only stores are in
+      // flight, and stores complete in order: all but this step's store
+      // (present when one of the wave's lanes has a row) are complete, the
+      // previous step's included.
+      if (__builtin_amdgcn_ballot_w64(has_row) != 0)
+        __builtin_amdgcn_s_waitcnt(vmcnt_only(1));
+      else
+        __builtin_amdgcn_s_waitcnt(vmcnt_only(0));
+    }
+    lds_barrier();  // ring entries written; every wave's previous-step stores complete
+    if (threadIdx.x == 0 && rel > 0)
+      __hip_atomic_store(&progress[c], tslev[rel - 1], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // the last step: its stores complete, then its level published
+  if (!pf) __builtin_amdgcn_s_waitcnt(0);
+  lds_barrier();
+  if (threadIdx.x == 0 && ns > 0 && !__builtin_amdgcn_readfirstlane(failed))
+    __hip_atomic_store(&progress[c], tslev[ns - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (pf) __builtin_amdgcn_s_waitcnt(0);  // no copy into LDS outlives the workgroup
+}
+
+// xc[t] = x[row of schedule position t]: the right-hand side in the chunked
+// schedule's order, for k_tri_chunks' prefetch.
+__global__ __launch_bounds__(kBlock) void k_ch_gather(int n, const int *__restrict__ crow,
+                                                      const double *__restrict__ x,
+                                                      double *__restrict__ xc,
+                                                      const int *__restrict__ done) {
+  if (done && *done) return;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock)
+    xc[t] = x[crow[t]];
+}
+
+// The records of the chunked schedule for one set of values (k_tri_chunks).
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_ch_pack(int n, const int *__restrict__ cmap,
+                                                    const int4 *__restrict__ desc,
+                                                    const int *__restrict__ ja,
+                                                    const double *__restrict__ a,
+                                                    const int *__restrict__ code,
+                                                    char *__restrict__ rec) {
+  using R = ChRec<W>;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    int *ri = reinterpret_cast<int *>(rec + (size_t)t * R::bytes);
+    double *rd = reinterpret_cast<double *>(rec + (size_t)t * R::bytes);
+    const int4 d = desc[cmap[t]];
+#pragma unroll
+    for (int j = 0; j < W; ++j) ri[j] = code[(size_t)t * W + j];
+    ri[R::row] = d.x;
+    int k = 0;
+    double piv = 0.0;
+    for (int e = d.y; e < d.z; ++e) {
+      if (ja[e] == d.x)
+        piv = a[e];
+      else if (k < W)
+        rd[R::val + k++] = a[e];
+    }
+    for (; k < W; ++k) rd[R::val + k] = 0.0;
+    rd[R::piv] = piv;
+  }
+}
+
+int *ch_progress() {
+  static DevBufPtr buf;
+  if (!buf) buf = dev_alloc(kChMax * sizeof(int));
+  return buf && buf->p ? buf->as<int>() : nullptr;
+}
+
 int *sf_state() {  // [ticket, error, barrier arrivals, pad]
   static DevBufPtr buf;
   if (!buf) {
@@ -415,6 +734,242 @@ void forget_tri_plans(const void *p) {
     else
       ++it;
   }
+}
+
+// ------------------------------------------------ chunked schedule (host)
+//
+// A partition of the rows into at most one chunk per CU is chosen among a few
+// candidates by simulating the pipeline: every step of a chunk costs one unit
+// and every wait on another chunk kHopCost more (a value crossing between
+// workgroups goes through memory).  Candidates: contiguous ranges in solve
+// order, and for matrices whose column offsets take few values (grids in
+// lexicographic order) tiles over two of the offset strides, which keep the
+// long dependency chains inside one chunk (e.g. the 256^3 grid's chains along
+// z: 16 x 16 x 256 tiles cross 30 chunk boundaries on the critical path
+// against 255 for z-slabs).
+constexpr int kHopCost = 6;
+
+struct ChunkSched {
+  int G = 0;
+  std::vector<int> cbase, crow, cmap;  // rows in (chunk, level, row) order
+  std::vector<int> cstep, sptr, slev, wptr, wchk, wlev;
+  long long cost = -1;
+};
+
+// Steps and pruned waits of a partition; false when a chunk's step holds
+// more than max_step rows.
+static bool sched_build(ChunkSched &S, const std::vector<int> &chunk_of, int G,
+                        const std::vector<int> &ia, const std::vector<int> &ja,
+                        const std::vector<int> &level, const std::vector<int> &order,
+                        int max_step) {
+  const int n = (int)chunk_of.size();
+  S.G = G;
+  S.cbase.assign((size_t)G + 1, 0);
+  for (int i = 0; i < n; ++i) S.cbase[(size_t)chunk_of[(size_t)i] + 1]++;
+  for (int c = 0; c < G; ++c) S.cbase[(size_t)c + 1] += S.cbase[(size_t)c];
+  S.crow.assign((size_t)n, 0);
+  S.cmap.assign((size_t)n, 0);
+  std::vector<int> fill(S.cbase.begin(), S.cbase.end() - 1);
+  for (int t = 0; t < n; ++t) {  // order is (level, row): bucket stably by chunk
+    const int i = order[(size_t)t];
+    const int pos = fill[(size_t)chunk_of[(size_t)i]]++;
+    S.crow[(size_t)pos] = i;
+    S.cmap[(size_t)pos] = t;
+  }
+  S.cstep.assign((size_t)G + 1, 0);
+  S.sptr.clear();
+  S.slev.clear();
+  S.wptr.clear();
+  S.wchk.clear();
+  S.wlev.clear();
+  std::vector<int> running((size_t)G, -1), req((size_t)G, -1), touched, prod;
+  for (int c = 0; c < G; ++c) {
+    S.cstep[(size_t)c] = (int)S.slev.size();
+    prod.clear();
+    for (int t = S.cbase[(size_t)c]; t < S.cbase[(size_t)c + 1];) {
+      const int lv = level[(size_t)S.crow[(size_t)t]];
+      int u = t;
+      touched.clear();
+      while (u < S.cbase[(size_t)c + 1] && level[(size_t)S.crow[(size_t)u]] == lv) {
+        const int i = S.crow[(size_t)u];
+        for (int e = ia[i]; e < ia[i + 1]; ++e) {
+          const int d = ja[e];
+          if (d == i) continue;
+          const int cd = chunk_of[(size_t)d];
+          if (cd == c) continue;
+          if (req[(size_t)cd] < 0) touched.push_back(cd);
+          req[(size_t)cd] = std::max(req[(size_t)cd], level[(size_t)d]);
+        }
+        ++u;
+      }
+      if (u - t > max_step) return false;
+      S.sptr.push_back(t);
+      S.slev.push_back(lv);
+      S.wptr.push_back((int)S.wchk.size());
+      std::sort(touched.begin(), touched.end());
+      for (int cd : touched) {
+        if (req[(size_t)cd] > running[(size_t)cd]) {
+          if (running[(size_t)cd] < 0) prod.push_back(cd);
+          running[(size_t)cd] = req[(size_t)cd];
+          S.wchk.push_back(cd);
+          S.wlev.push_back(req[(size_t)cd]);
+        }
+        req[(size_t)cd] = -1;
+      }
+      t = u;
+    }
+    for (int cd : prod) running[(size_t)cd] = -1;
+  }
+  const int nsteps = (int)S.slev.size();
+  S.cstep[(size_t)G] = nsteps;
+  S.sptr.push_back(n);
+  S.wptr.push_back((int)S.wchk.size());
+  for (int c = 0; c < G; ++c) {  // the kernel's LDS tables
+    const int k0 = S.cstep[(size_t)c], k1 = S.cstep[(size_t)c + 1];
+    if (k1 - k0 > kChSteps || S.wptr[(size_t)k1] - S.wptr[(size_t)k0] > kChWaits) return false;
+  }
+  // pipeline simulation, steps taken in level order (every wait refers to a
+  // lower level, so its producer step is already timed)
+  int nlev = 0;
+  for (int v : S.slev) nlev = std::max(nlev, v + 1);
+  std::vector<int> lcnt((size_t)nlev + 1, 0), byl((size_t)nsteps), owner((size_t)nsteps);
+  for (int c = 0; c < G; ++c)
+    for (int k = S.cstep[(size_t)c]; k < S.cstep[(size_t)c + 1]; ++k) owner[(size_t)k] = c;
+  for (int k = 0; k < nsteps; ++k) lcnt[(size_t)S.slev[(size_t)k] + 1]++;
+  for (int l = 0; l < nlev; ++l) lcnt[(size_t)l + 1] += lcnt[(size_t)l];
+  for (int k = 0; k < nsteps; ++k) byl[(size_t)lcnt[(size_t)S.slev[(size_t)k]]++] = k;
+  std::vector<long long> tdone((size_t)nsteps, 0);
+  long long cost = 0;
+  for (int k : byl) {
+    const int c = owner[(size_t)k];
+    long long t0 = k > S.cstep[(size_t)c] ? tdone[(size_t)k - 1] : 0;
+    for (int q = S.wptr[(size_t)k]; q < S.wptr[(size_t)k + 1]; ++q) {
+      const int cd = S.wchk[(size_t)q], L = S.wlev[(size_t)q];
+      // the producer's first step at a level >= L
+      const int *b0 = S.slev.data() + S.cstep[(size_t)cd], *b1 = S.slev.data() + S.cstep[(size_t)cd + 1];
+      const int kk = (int)(std::lower_bound(b0, b1, L) - S.slev.data());
+      t0 = std::max(t0, tdone[(size_t)kk] + kHopCost);
+    }
+    tdone[(size_t)k] = t0 + 1;
+    cost = std::max(cost, t0 + 1);
+  }
+  S.cost = cost;
+  return true;
+}
+
+// The chunked schedule of a triangle whose rows hold at most 8 entries.
+static void tri_chunks_build(TriPlan &p, const std::vector<int> &ia, const std::vector<int> &ja,
+                             const std::vector<int> &level, const std::vector<int> &order) {
+  const int n = p.n;
+  if (n <= 0 || p.max_row > 5 || !p.has_diag) return;
+  const int Gmax = std::min({device_cus(), kChMax, std::max(1, n / kChMinRows)});
+  const bool lower = p.lower;
+  auto pos_of = [&](int i) { return lower ? i : n - 1 - i; };
+  std::vector<std::pair<std::vector<int>, int>> cands;
+  {  // contiguous ranges in solve order
+    int C = (n + Gmax - 1) / Gmax;
+    C = (C + 63) / 64 * 64;
+    const int G = (n + C - 1) / C;
+    std::vector<int> co((size_t)n);
+    for (int i = 0; i < n; ++i) co[(size_t)i] = pos_of(i) / C;
+    cands.emplace_back(std::move(co), G);
+  }
+  {  // tiles over two offset strides
+    std::vector<int> offs;
+    for (int i = 0; i < n && offs.size() <= 16; ++i)
+      for (int e = ia[i]; e < ia[i + 1]; ++e) {
+        const int d = ja[e];
+        if (d == i) continue;
+        const int o = std::abs(i - d);
+        if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
+      }
+    if (offs.size() <= 16) {
+      std::sort(offs.begin(), offs.end());
+      for (size_t x = 0; x < offs.size(); ++x)
+        for (size_t y = x + 1; y < offs.size(); ++y) {
+          const int Oa = offs[x], Ob = offs[y];
+          if (Oa < 2 || Ob % Oa || n % Ob) continue;
+          const int nb = Ob / Oa;  // extent of the second coordinate
+          for (int nI = 1; nI <= Gmax; nI *= 2) {
+            const int nJ = Gmax / nI;
+            if (nI > Oa || nJ > nb || nI * nJ < 2) continue;
+            const int Ti = (Oa + nI - 1) / nI, Tj = (nb + nJ - 1) / nJ;
+            const int gi = (Oa + Ti - 1) / Ti, gj = (nb + Tj - 1) / Tj;
+            std::vector<int> co((size_t)n);
+            for (int r = 0; r < n; ++r) {
+              const int ps = pos_of(r);
+              co[(size_t)r] = ((ps / Oa) % nb / Tj) * gi + (ps % Oa) / Ti;
+            }
+            cands.emplace_back(std::move(co), gi * gj);
+          }
+        }
+    }
+  }
+  ChunkSched best, cur;
+  size_t best_i = 0;
+  for (size_t ci = 0; ci < cands.size(); ++ci) {
+    if (!sched_build(cur, cands[ci].first, cands[ci].second, ia, ja, level, order, kChStepRows))
+      continue;
+    if (best.cost < 0 || cur.cost < best.cost) {
+      std::swap(best, cur);
+      best_i = ci;
+    }
+  }
+  if (best.cost < 0) return;
+  if (getenv("LAP_SCHED_LOG"))
+    fprintf(stderr, "[lapgfx] %s triangle n=%d: chunked schedule %zu of %zu, %d chunks, %zu steps, "
+            "%zu waits, simulated length %lld (one launch per level: %d)\n",
+            lower ? "lower" : "upper", n, best_i, cands.size(), best.G, best.slev.size(),
+            best.wchk.size(), best.cost, p.nlevels);
+  const std::vector<int> &chunk_of = cands[best_i].first;
+  const int G = best.G;
+  // dependency codes of the off-diagonal entries, records in schedule order:
+  // -1 padding, >= 0 a row read from y, <= -2 the LDS ring slot -(code + 2)
+  // of a row of this chunk solved fewer than kChRing steps earlier
+  const int W = 4;  // off-diagonal entries per record
+  std::vector<int> lstep((size_t)n), lpos((size_t)n);  // local step and position of each row
+  for (int c = 0; c < G; ++c)
+    for (int k = best.cstep[(size_t)c]; k < best.cstep[(size_t)c + 1]; ++k)
+      for (int t = best.sptr[(size_t)k]; t < best.sptr[(size_t)k + 1]; ++t) {
+        lstep[(size_t)best.crow[(size_t)t]] = k - best.cstep[(size_t)c];
+        lpos[(size_t)best.crow[(size_t)t]] = t - best.sptr[(size_t)k];
+      }
+  std::vector<int> code((size_t)n * W, -1);
+  for (int t = 0; t < n; ++t) {
+    const int i = best.crow[(size_t)t];
+    int k = 0;
+    for (int e = ia[i]; e < ia[i + 1]; ++e) {
+      const int d = ja[e];
+      if (d == i) continue;  // the pivot is kept apart
+      int cd = d;
+      if (chunk_of[(size_t)d] == chunk_of[(size_t)i] &&
+          lstep[(size_t)i] - lstep[(size_t)d] < kChRing)
+        cd = -(2 + (lstep[(size_t)d] % kChRing) * kChStepRows + lpos[(size_t)d]);
+      code[(size_t)t * W + k++] = cd;
+    }
+  }
+  // the progress counters and the error flag, allocated here (never inside
+  // a graph capture of the solve)
+  if (!ch_progress() || !sf_state()) return;
+  if (best.wchk.empty()) {  // keep the device arrays non-empty
+    best.wchk.push_back(0);
+    best.wlev.push_back(-1);
+  }
+  TriPlan::Chunks &ch = p.ch;
+  ch.nchunks = G;
+  ch.nsteps = (int)best.slev.size();
+  ch.nwaits = best.wptr.back();
+  ch.w = W;
+  ch.cost = best.cost;
+  ch.cstep = dev_upload(best.cstep);
+  ch.sptr = dev_upload(best.sptr);
+  ch.slev = dev_upload(best.slev);
+  ch.wptr = dev_upload(best.wptr);
+  ch.wchk = dev_upload(best.wchk);
+  ch.wlev = dev_upload(best.wlev);
+  ch.crow = dev_upload(best.crow);
+  ch.cmap = dev_upload(best.cmap);
+  ch.code = dev_upload(code);
 }
 
 // Host-side level analysis of a triangular CSR pattern.
@@ -494,6 +1049,7 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
   cp.push_back(n);
   p->nchunks = (int)cp.size() - 1;
   p->chunk_ptr = dev_upload(cp);
+  tri_chunks_build(*p, ia, ja, level, order);
   return p;
 }
 
@@ -547,13 +1103,69 @@ void tri_ell_build(const TriPlan &p, const int *ja, const double *a, TriEll &e) 
     k_ell_pack<8><<<g, kBlock, 0, stream()>>>(p.n, p.desc->as<int4>(), ja, a, e.ja->as<int>(),
                                               e.a->as<double>());
   e.w = w;
+  if (p.ch.nchunks > 0) {  // the records of the chunked schedule for these values
+    const int cw = p.ch.w;
+    const size_t rb = (size_t)(12 * cw + 32);
+    // kBlock records of padding: the prefetch copies whole slots
+    e.crec = dev_alloc(rb * ((size_t)p.n + kBlock));
+    e.xc = dev_alloc(sizeof(double) * ((size_t)p.n + kBlock));
+    if (!e.crec || !e.crec->p || !e.xc || !e.xc->p) return;
+    if (cw == 4)
+      k_ch_pack<4><<<g, kBlock, 0, stream()>>>(p.n, p.ch.cmap->as<int>(), p.desc->as<int4>(), ja,
+                                               a, p.ch.code->as<int>(), e.crec->as<char>());
+    else
+      k_ch_pack<8><<<g, kBlock, 0, stream()>>>(p.n, p.ch.cmap->as<int>(), p.desc->as<int4>(), ja,
+                                               a, p.ch.code->as<int>(), e.crec->as<char>());
+  }
+}
+
+// Which path tri_solve_launch takes.
+enum TriPath { TRI_CHUNKS, TRI_LEVELS_ELL, TRI_BARRIER, TRI_SYNCFREE, TRI_LEVELS };
+
+static TriPath tri_path(const TriPlan &p, const TriEll *ell, bool in_place) {
+  const int mode = options().sptrsv_mode;
+  if (mode == 3 && ell && ell->w && ell->crec && ell->xc && p.ch.nchunks > 0 && p.ch.w == 4 &&
+      sf_state() && ch_progress())
+    return TRI_CHUNKS;
+  if (ell && ell->w && (mode == 0 || mode == 3)) return TRI_LEVELS_ELL;
+  // in place (x == y) needs x intact while y is filled: level launches then
+  if (mode == 2 && !in_place && p.nlevels > 0 && sf_state()) return TRI_BARRIER;
+  if (mode == 1 && !in_place && p.n > 0 && sf_state()) return TRI_SYNCFREE;
+  return TRI_LEVELS;
+}
+
+int tri_launches(const TriPlan &p, const TriEll *ell, bool in_place) {
+  switch (tri_path(p, ell, in_place)) {
+    case TRI_CHUNKS: return 3;
+    case TRI_BARRIER: return 2;
+    case TRI_SYNCFREE: return 2;
+    default: return p.nlevels;
+  }
 }
 
 void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const double *a,
                       const double *diag, const double *x, double *y, const int *done,
                       const TriEll *ell) {
   const int *order = p.order->as<int>();
-  if (ell && ell->w && options().sptrsv_mode == 0) {
+  const TriPath path = tri_path(p, ell, x == y);
+  if (path == TRI_CHUNKS) {
+    int *st = sf_state(), *prog = ch_progress();
+    {
+      const TriPlan::Chunks &ch = p.ch;
+      k_ch_reset<<<(ch.nchunks + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(ch.nchunks, prog,
+                                                                             done);
+      k_ch_gather<<<grid_for((size_t)p.n, 4, kBlock, 4096), kBlock, 0, stream()>>>(
+          p.n, ch.crow->as<int>(), x, ell->xc->as<double>(), done);
+#define LAP_CH_ARGS                                                                            \
+  ch.cstep->as<int>(), ch.sptr->as<int>(), ch.slev->as<int>(), ch.wptr->as<int>(),             \
+      ch.wchk->as<int>(), ch.wlev->as<int>(), ell->crec->as<char>(),                           \
+      ell->xc->as<double>(), y, prog, st, done
+      k_tri_chunks<4, 3, 4><<<ch.nchunks, kBlock + kWave, 0, stream()>>>(LAP_CH_ARGS);
+#undef LAP_CH_ARGS
+      return;
+    }
+  }
+  if (path == TRI_LEVELS_ELL) {
     for (int l = 0; l < p.nlevels; ++l) {
       const int l0 = p.level_ptr[l], cnt = p.level_ptr[l + 1] - l0;
       const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock);
@@ -569,10 +1181,9 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
     }
     return;
   }
-  // in place (x == y) needs x intact while y is filled: level launches then
-  if (options().sptrsv_mode == 2 && x != y && p.nlevels > 0) {
+  if (path == TRI_BARRIER) {
     int *st = sf_state();
-    if (st) {
+    {
       k_tb_reset<<<1, kWave, 0, stream()>>>(st, done);
       const unsigned g = (unsigned)std::max(
           1, std::min((p.max_level_rows + kBlock - 1) / kBlock, device_cus()));
@@ -581,8 +1192,8 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
       return;
     }
   }
-  int *st = (options().sptrsv_mode == 1 && x != y) ? sf_state() : nullptr;
-  if (st && p.n > 0) {
+  if (path == TRI_SYNCFREE) {
+    int *st = sf_state();
     const unsigned gf = grid_for((size_t)p.n, 4, kBlock, 2048);
     k_sf_fill<<<gf, kBlock, 0, stream()>>>(p.n, y, st, done);
     const int per_cu = std::max(1, std::min(16, options().sf_wg_per_cu));

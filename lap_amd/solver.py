@@ -112,6 +112,14 @@ def ichol_factor(A: DeviceCSR) -> torch.Tensor:
     return vals
 
 
+def ichol_refactor(A: DeviceCSR, vals: torch.Tensor) -> None:
+    """Factors A's values held in `vals` in place again (initialize_ichol on
+    the same buffer), e.g. after the matrix values changed."""
+    _sync()
+    L.load().initialize_ichol(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(vals))
+    L.check()
+
+
 # -------------------------------------------------------- preconditioners
 
 @dataclass

@@ -8,6 +8,16 @@ Two independent routes per case:
     RHS file written here (iteration count and the summary's 17-digit final
     residual), which also exercises its MM reader and setup.
 Run in this container:  python tests/golden/make_golden.py [--big]
+
+`--spread` adds to every existing fixture the spread of the reference CPU
+path's own arithmetic on that input: the oracle's OpenMP restatement of
+openmp_blas.c (dot products summed by 2..16 threads, SpMV on 16 threads,
+the racy shared column index made private) run for exactly the fixture's
+iteration count (tol 0, maxit = iterations - 1), compared with the serial
+solution through the same samples and sketch the GPU tests use
+(`omp_spread`) and residual norm by residual norm (`omp_spread_history`,
+relative to each norm).  The GPU equal-iteration tests hold x to 1e-10
+relative and the history to 1e-7, or to twice these spreads where larger.
 """
 from __future__ import annotations
 
@@ -78,6 +88,60 @@ def case(name, A, b, kind, tol, maxit=25000, m=1, k=1, full_x=False, binary=True
     print(f"{name}: {kind} it={it} flag={flag} res={hist[-1]:.3e}")
 
 
+def fixture_problem(name):
+    """The system of a fixture, from its name."""
+    if name.startswith("c1_"):
+        A = grid_laplacian([128, 128], 1e-8)
+        return A, rhs_uniform(A.n, 12345)
+    if name.startswith("g32_"):
+        A = grid_laplacian([32, 32, 32], 1e-8)
+        return A, rhs_normal(A.n, 12345)
+    if name.startswith("c2_"):
+        A = grid_laplacian([256, 256, 256], 1e-8)
+        return A, rhs_normal(A.n, 12345)
+    from lap_amd.problems import chung_lu
+    A = chung_lu(1 << 22)
+    return A, rhs_normal(A.n, 12345)
+
+
+def solution_dev(x, g, centre: bool):
+    """Relative deviation of x from a fixture's solution through its 512
+    samples and 16-row sketch (means removed when `centre`); the larger."""
+    n = x.shape[0]
+    step = max(1, n // 512)
+    rng = np.random.default_rng(777)
+    ones = np.array([(rng.integers(0, 2, n) * 2.0 - 1.0).sum() for _ in range(SKETCH_ROWS)])
+    xm, gm = (x.mean(), float(g["xmean"])) if centre else (0.0, 0.0)
+    xs, gs = x[::step] - xm, g["x_sample"] - gm
+    sk, gk = sketch(x) - xm * ones, g["sketch"] - gm * ones
+    return max(np.linalg.norm(xs - gs) / np.linalg.norm(gs),
+               np.linalg.norm(sk - gk) / np.linalg.norm(gk))
+
+
+def add_spread(names):
+    orc = oracle.Oracle()
+    for name in names:
+        path = os.path.join(OUT, name + ".npz")
+        g = dict(np.load(path, allow_pickle=False))
+        A, b = fixture_problem(name)
+        assert input_hash(A, b) == str(g["input_sha256"]), name
+        it = int(g["iterations"])
+        threads = (8,) if name.startswith(("c2_", "c3_")) else (2, 4, 8, 16)
+        devs, hdevs = [], []
+        for th in threads:
+            x, its, _, h = orc.cg(A, b, str(g["kind"]), tol=0.0, maxit=it - 1, m=int(g["m"]),
+                                  k=int(g["k"]), omp=(16, th))
+            assert its == it, (name, its, it)
+            devs.append(solution_dev(x, g, centre=not name.startswith("c3_")))
+            hdevs.append(float(np.max(np.abs(h - g["history"]) / g["history"])))
+        g["omp_spread"] = max(devs)
+        g["omp_spread_history"] = max(hdevs)
+        g["omp_spread_threads"] = np.array(threads)
+        np.savez_compressed(path, **g)
+        print(f"{name}: OpenMP-vs-serial spread {max(devs):.2e}, residual history "
+              f"{max(hdevs):.2e} (threads {threads})", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also the 256^3 Jacobi case (slow)")
@@ -85,7 +149,15 @@ def main():
                     help="only the 256^3 GS_std and IC0 cases (tens of minutes)")
     ap.add_argument("--power-law", action="store_true",
                     help="only the 2^22 power-law (Chung-Lu) Jacobi case (minutes)")
+    ap.add_argument("--spread", nargs="*", metavar="NAME",
+                    help="add the OpenMP-vs-serial spread to these fixtures (default: all)")
     args = ap.parse_args()
+    if args.spread is not None:
+        import glob
+        names = args.spread or sorted(os.path.basename(p)[:-4]
+                                      for p in glob.glob(os.path.join(OUT, "*.npz")))
+        add_spread(names)
+        return
     if not oracle.have_ref():
         sys.exit("build oracle/_ref first: bash oracle/build_ref.sh")
     if args.power_law:

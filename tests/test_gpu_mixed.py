@@ -124,3 +124,50 @@ def test_mixed_power_law_fp64_vectors(orc, kind, fp64_inner_vectors):
     assert m.iterations <= d.iterations + 5
     assert m.refinements <= 6
     assert err <= 1e3 * 1e-10
+
+
+@pytest.mark.parametrize("vectors", [0, 1])
+@pytest.mark.parametrize("kind", ["line_jacobi", "ichol"])
+def test_mixed_sees_values_changed_in_place(orc, kind, vectors):
+    """The caller may change A's values, d_r and the IC0 factor in place
+    between solves (same addresses).  The second solve must use the new
+    values: A is scaled by 3 and re-shifted in place, d_r recomputed into the
+    same buffer and the factor recomputed into the same ichol_vals, and the
+    second mixed solve is checked against the FP64 oracle on the new system
+    (true residual, solution within 1e3 * tol)."""
+    print(f"[case] mixed in-place values kind={kind} vectors={vectors}", flush=True)
+    import torch
+    prev = S.set_option("mixed_vectors", vectors)
+    try:
+        Ah = CASES["graph_big"]()
+        b = rhs_normal(Ah.n, 41)
+        A = S.DeviceCSR.from_csr(Ah)
+        P = S.make_preconditioner(A, Ah, kind)
+        m1 = S.pcg_mixed(A, S.to_dev(b), P, tol=1e-10, maxit=5000)
+        assert m1.flag == 0
+        # new values at the same addresses: A2 = 3 A + 0.5 I
+        a2 = 3.0 * Ah.a.copy()
+        rows = np.repeat(np.arange(Ah.n), np.diff(Ah.ia))
+        a2[Ah.ja == rows] += 0.5
+        from lap_amd.problems import CSR
+        A2 = CSR(Ah.n, Ah.ia.copy(), Ah.ja.copy(), a2)
+        A.a.copy_(torch.from_numpy(a2))
+        d2 = a2[Ah.ja == rows]
+        P.keep["d"].copy_(torch.from_numpy(d2))
+        P.keep["d_r"].copy_(torch.from_numpy(1.0 / d2))
+        if kind == "ichol":
+            ichol = P.keep["ichol_vals"]
+            ichol.copy_(A.a)
+            S.ichol_refactor(A, ichol)
+        m2 = S.pcg_mixed(A, S.to_dev(b), P, tol=1e-10, maxit=5000)
+        x = m2.x.cpu().numpy()
+        true = np.linalg.norm(b - orc.matvec(A2, x))
+        xo, ito, _, _ = orc.cg(A2, b, kind, tol=1e-10, maxit=5000)
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        print(f"[diag] mixed in-place {kind}/vectors {vectors}: it {m2.iterations} (oracle {ito}),"
+              f" true rel res {true / np.linalg.norm(b):.2e}, err vs oracle {err:.2e}",
+              flush=True)
+        assert m2.flag == 0 and true < 1e-10 * np.linalg.norm(b) * 1.01
+        assert err <= 1e3 * 1e-10
+    finally:
+        S.set_option("mixed_vectors", prev)

@@ -7,8 +7,12 @@ Bars (stated per check):
     iterate x matches the oracle's to 1e-9 relative: only the dot products'
     summation order differs, every vector update has the reference's
     rounding;
-  * at convergence the solution matches the reference's to 1e-10 relative on
-    the complement of the constant vector.  The systems are graph Laplacians
+  * run for exactly the reference's iteration count (tol 0, maxit =
+    iterations - 1: cg.c stops when iter > maxit), the solution matches the
+    reference's to 1e-10 relative on the complement of the constant vector,
+    or to twice the reference CPU path's own OpenMP-vs-serial spread on that
+    input where that is larger (stored in the fixture as `omp_spread`; only
+    c1 it_jacobi, 1.7e-10, exceeds 1e-10).  The systems are graph Laplacians
     + 1e-8*I, where the constant vector is a near-null mode: rounding of b's
     mean (~1e-17) moves x along it by ~1e-9 in any two implementations, so the
     comparison removes the mean first;
@@ -16,6 +20,8 @@ Bars (stated per check):
 """
 import glob
 import os
+
+import re
 
 import numpy as np
 import pytest
@@ -54,6 +60,70 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
+def solution_dev(x, g, centre=True):
+    """Relative deviation of x from a fixture's solution through its 512
+    evenly spaced samples and its 16-row +-1 sketch (means removed when
+    `centre`), as (samples, sketch)."""
+    step = max(1, x.shape[0] // 512)
+    sk, ones = _sketch(x)
+    xm, gm = (x.mean(), float(g["xmean"])) if centre else (0.0, 0.0)
+    ds = rel(x[::step] - xm, g["x_sample"] - gm)
+    dk = rel(sk - xm * ones, g["sketch"] - gm * ones)
+    return ds, dk
+
+
+def solution_bar(g):
+    """1e-10, or twice the reference CPU path's own OpenMP-vs-serial spread
+    on the fixture's input where that is larger."""
+    return max(1e-10, 2.0 * float(g["omp_spread"]) if "omp_spread" in g else 0.0)
+
+
+def history_bar(g):
+    """Whole-history bar: every residual norm within 1e-7 of the reference's
+    (relative to that norm), or within twice the reference CPU path's own
+    OpenMP-vs-serial spread on that input where larger.  The dot products are
+    summed in another order, so the recurrences part in the last bits and
+    the deviation grows as the residual falls; on the 128^2 IC0 case the
+    reference's own builds already differ by 2.5e-2 in the last norms
+    (2e-9 of the initial one)."""
+    return max(1e-7, 2.0 * float(g["omp_spread_history"]) if "omp_spread_history" in g else 0.0)
+
+
+def equal_iteration_parity(A, b, P, g, centre=True):
+    """Runs the solve for exactly the fixture's iteration count and checks the
+    solution and the whole residual history against it."""
+    it_ref = int(g["iterations"])
+    res = S.pcg(A, S.to_dev(b), P, tol=0.0, maxit=it_ref - 1)
+    assert (res.iterations, res.flag) == (it_ref, 1), (res.iterations, res.flag, it_ref)
+    x = res.x.cpu().numpy()
+    ds, dk = solution_dev(x, g, centre)
+    hdev = float(np.max(np.abs(res.history - g["history"]) / g["history"]))
+    bar = solution_bar(g)
+    return x, ds, dk, hdev, bar
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_pcg_equal_iterations_match_golden(name):
+    """x after exactly the reference's iteration count: within 1e-10 of the
+    reference's (or twice its OpenMP-vs-serial spread), the residual history
+    within history_bar."""
+    print(f"[case] golden equal-iteration {name}", flush=True)
+    g = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+    Ah, b = problem_for(name)
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, str(g["kind"]), m=int(g["m"]), k=int(g["k"]))
+    x, ds, dk, hdev, bar = equal_iteration_parity(A, b, P, g)
+    msg = f"samples {ds:.2e}, sketch {dk:.2e}"
+    if "x" in g:
+        dx = rel(centred(x), centred(g["x"]))
+        msg += f", full x {dx:.2e}"
+        assert dx <= bar
+    print(f"[diag] {name} equal-iteration: {msg} (bar {bar:.1e}); history {hdev:.2e}",
+          flush=True)
+    assert ds <= bar and dk <= bar
+    assert hdev <= history_bar(g)
+
+
 @pytest.mark.parametrize("name", golden_cases())
 def test_pcg_matches_golden(name):
     print(f"[case] golden={name}", flush=True)
@@ -72,12 +142,13 @@ def test_pcg_matches_golden(name):
     hdev = np.max(np.abs(h[:k] - g["history"][:k]) / g["history"][:k])
     x = res.x.cpu().numpy()
     print(f"[diag] {name}: it {res.iterations} vs {it_ref}, max hist rel dev {hdev:.2e}", flush=True)
-    assert hdev <= 1e-2
+    assert hdev <= history_bar(g)
     assert res.true_res <= float(g["tol"]) * res.res0 * 1.5
-    if "x" in g:
-        dx = rel(centred(x), centred(g["x"]))
-        print(f"[diag] {name}: solution rel dev (mean removed) {dx:.2e}", flush=True)
-        assert dx <= 1e-6
+    if res.iterations == it_ref:
+        ds, dk = solution_dev(x, g)
+        print(f"[diag] {name}: converged solution samples {ds:.2e}, sketch {dk:.2e}",
+              flush=True)
+        assert ds <= solution_bar(g) and dk <= solution_bar(g)
 
 
 @pytest.mark.parametrize("kind", ["line_jacobi", "GS_std", "ichol", "it_jacobi", "GS_it2", "none"])
@@ -244,7 +315,15 @@ def test_pcg_256cubed_matches_reference(c2_system, kind):
     ref_c = g["sketch"] - float(g["xmean"]) * ones
     dk = np.linalg.norm(sk_c - ref_c) / np.linalg.norm(ref_c)
     print(f"[diag] 256^3 {kind}: sample rel dev {ds:.2e}, sketch rel dev {dk:.2e}", flush=True)
-    assert ds <= 1e-6 and dk <= 1e-6
+    # converged one iteration apart (IC0: 841 against 842) the solutions differ
+    # by one CG step; at the same count they meet the equal-iteration bar
+    bar_conv = solution_bar(g) if res.iterations == it_ref else 1e-6
+    assert ds <= bar_conv and dk <= bar_conv
+    x, ds, dk, hdev, bar = equal_iteration_parity(A, b, P, g)
+    print(f"[diag] 256^3 {kind} equal-iteration ({it_ref}): samples {ds:.2e}, sketch {dk:.2e} "
+          f"(bar {bar:.1e}); history {hdev:.2e}", flush=True)
+    assert ds <= bar and dk <= bar
+    assert hdev <= history_bar(g)
 
 
 def test_pcg_power_law_matches_reference():
@@ -277,6 +356,11 @@ def test_pcg_power_law_matches_reference():
     dk = np.linalg.norm(sk - g["sketch"]) / np.linalg.norm(g["sketch"])
     print(f"[diag] power law 2^22: sample rel dev {ds:.2e}, sketch rel dev {dk:.2e}", flush=True)
     assert ds <= 1e-10 and dk <= 1e-10
+    _, ds, dk, hdev, bar = equal_iteration_parity(A, b, P, g, centre=False)
+    print(f"[diag] power law 2^22 equal-iteration ({int(g['iterations'])}): samples {ds:.2e}, "
+          f"sketch {dk:.2e} (bar {bar:.1e}); history {hdev:.2e}", flush=True)
+    assert ds <= bar and dk <= bar
+    assert hdev <= history_bar(g)
     # the mixed-precision solves of the same system against the reference's
     # FP64 solution: within 1e3 * tol, the mixed solver's bar
     for vectors in (1, 0):
@@ -324,13 +408,14 @@ def test_jacobi_variants_are_bitwise_identical(pname):
         S.set_option("jacobi_variant", prev)
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("kind", ["ichol", "GS_std"])
 @pytest.mark.parametrize("pname", ["grid3d_16", "graph_big"])
 def test_sync_free_trisolve_pcg_is_bitwise_the_level_scheduled_one(kind, pname, mode):
-    """sptrsv_mode 1 (one sync-free launch per triangular solve) and 2 (one
-    launch, a grid barrier per level) give the level-scheduled solve's bits,
-    so whole PCG runs agree exactly."""
+    """sptrsv_mode 1 (one sync-free launch per triangular solve), 2 (one
+    launch, a grid barrier per level) and 3 (one persistent launch over
+    chunks of rows) give the level-scheduled solve's bits, so whole PCG runs
+    agree exactly."""
     print(f"[case] one-launch trisolve pcg mode={mode} kind={kind} problem={pname}", flush=True)
     Ah = PROBLEMS[pname]()
     A = S.DeviceCSR.from_csr(Ah)
@@ -379,12 +464,60 @@ def test_packed_level_rows_are_bitwise_the_csr_level_kernel(kind, pname):
     A = S.DeviceCSR.from_csr(Ah)
     P = S.make_preconditioner(A, Ah, kind)
     b = S.to_dev(rhs_normal(Ah.n, 79))
+    prev = S.set_option("sptrsv_mode", 0)
     try:
-        S.set_option("sptrsv_mode", 0)
         r0 = S.pcg(A, b, P, tol=1e-10, maxit=5000)
         S.set_option("sptrsv_mode", 2)
         r2 = S.pcg(A, b, P, tol=1e-10, maxit=5000)
     finally:
-        S.set_option("sptrsv_mode", 0)
+        S.set_option("sptrsv_mode", prev)
     assert r0.flag == 0 and (r0.iterations, r0.flag) == (r2.iterations, r2.flag)
     assert np.array_equal(r0.history, r2.history) and torch.equal(r0.x, r2.x)
+
+
+def _chunk_cases():
+    from lap_amd.problems import grid_laplacian
+    return {
+        "grid3d_48": lambda: grid_laplacian([48, 48, 48], 1e-8),
+        "grid3d_64x64x40": lambda: grid_laplacian([64, 64, 40], 1e-8),
+        "grid2d_500x400": lambda: grid_laplacian([500, 400], 1e-8),
+        "nine_point_300x200": lambda: nine_point(300, 200),
+    }
+
+
+@pytest.mark.parametrize("kind", ["ichol", "GS_std"])
+@pytest.mark.parametrize("pname", list(_chunk_cases()))
+def test_chunked_trisolve_pcg_is_bitwise_the_level_launches(kind, pname, monkeypatch, capfd):
+    """sptrsv_mode 3 on systems large enough for many chunks (48^3: 54
+    chunks; 64x64x40: 80; 2D 500x400: 97; the 9-point stencil's 5-entry rows
+    packed 8 wide): a fixed number of iterations and a converged solve give
+    the level launches' x, history and iteration count bit for bit.  The
+    schedule log (LAP_SCHED_LOG) shows that both triangles got a chunked
+    schedule, so the comparison is not of the level launches with
+    themselves."""
+    print(f"[case] chunked trisolve kind={kind} problem={pname}", flush=True)
+    monkeypatch.setenv("LAP_SCHED_LOG", "1")
+    Ah = _chunk_cases()[pname]()
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, kind)
+    b = S.to_dev(rhs_normal(Ah.n, 81))
+    prev = S.set_option("sptrsv_mode", 0)
+    try:
+        for tol, maxit in ((1e-30, 30), (1e-10, 5000)):
+            S.set_option("sptrsv_mode", 0)
+            r0 = S.pcg(A, b, P, tol=tol, maxit=maxit)
+            S.set_option("sptrsv_mode", 3)
+            r3 = S.pcg(A, b, P, tol=tol, maxit=maxit)
+            print(f"[diag] chunked {kind}/{pname} tol {tol:g}: it {r3.iterations} vs "
+                  f"{r0.iterations}, {r3.gpu_ms:.1f} ms vs {r0.gpu_ms:.1f} ms", flush=True)
+            assert (r3.iterations, r3.flag) == (r0.iterations, r0.flag)
+            assert np.array_equal(r3.history, r0.history)
+            assert torch.equal(r3.x, r0.x)
+    finally:
+        S.set_option("sptrsv_mode", prev)
+    log = capfd.readouterr().err
+    chunks = {side: [int(m) for m in re.findall(side + r" triangle n=\d+: chunked schedule \d+ "
+                                                r"of \d+, (\d+) chunks", log)]
+              for side in ("lower", "upper")}
+    print(f"[diag] schedules {chunks}", flush=True)
+    assert chunks["lower"] and chunks["upper"] and min(chunks["lower"] + chunks["upper"]) > 1

@@ -18,7 +18,7 @@ OPTIONS = {
     "fuse_sweeps": ([0, 1], 2, 1),
     "spmv_format": ([0, 1, 2], 3, 0),
     "mixed_vectors": ([0, 1], 2, 0),
-    "sptrsv_mode": ([0, 1, 2], 3, 0),
+    "sptrsv_mode": ([0, 1, 2, 3], 4, 3),
     "sptrsv_long": ([0, 1], -1, 0),
     "spmm_wg_per_cu": ([1, 16], 17, None),
     "spmv_wg_per_cu": ([1, 16], 0, 8),
