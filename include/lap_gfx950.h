@@ -333,13 +333,25 @@ double lap_pcg_fused_timed(int n, int nnz, const int *ia, const int *ja, const d
  * the vector kernels), "graph_iters" (iterations per HIP graph launch),
  * "use_graph", "spmm_wg_per_cu" (block CG SpMM workgroups per CU),
  * "sptrsv_mode" (triangular solves: 0 one launch per level, 1 one
- * sync-free launch), "jacobi_variant" (the Jacobi PCG iteration: 0 three launches
+ * sync-free launch, 2 one launch with a grid barrier per level, 3, the
+ * default, one persistent launch over chunks of rows with point-to-point
+ * waits), "jacobi_variant" (the Jacobi PCG iteration: 0 three launches
  * with x updated beside r; 1, the default, three launches with x updated
  * beside p, which reads p once less; 2 two launches, the p update computed
  * in the next SpMV's gather).  Results do not depend on any of them except
  * vec_grid, which sets the dot-product summation order.  Returns the previous value,
  * or -1 for an unknown name. */
 int lap_set_option(const char *name, int value);
+
+/* No counterpart in the reference: hip_dot for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  *result (device memory) = v.w with
+ * hip_dot's kernels, so with its bits; n >= 0, any size (its partial sums
+ * never exceed 2048).  The partial sums live in a static 16 KiB device array,
+ * one per GPU, looked up at initialize_handles or at the first call on a
+ * device; no call allocates.  Calls that overlap on one GPU (other streams
+ * or processes) race on that array.  Returns LAP_OK or an error code. */
+int lap_dot_noalloc(const int n, const double *v, const double *w, double *result);
 
 #ifdef __cplusplus
 }

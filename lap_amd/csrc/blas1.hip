@@ -134,11 +134,70 @@ void launch_dot(int n, const double *v, const double *w, double *out) {
   k_dot_final<kBlock><<<1, kBlock, 0, stream()>>>((int)g, ws.partials, out);
 }
 
+// ------------------------------------------------ no-allocation dot product
+// lap_dot_noalloc keeps its partial sums in this array instead of the
+// library's workspace: one copy per GPU, held until the process ends.  On the
+// host `scratch` is a stand-in, not a device address, so host code only ever
+// passes the address hipGetSymbolAddress returned for the current device.
+constexpr int kNoAllocPartials = (int)kStreamGridCap;  // 2048 doubles, 16 KiB
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+constexpr int kNoAllocDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kNoAllocDevices];
+
+double *dot_scratch() {
+  int dev = 0;
+  if (!LAP_HIP(hipGetDevice(&dev))) return nullptr;
+  if (dev < 0 || dev >= kNoAllocDevices) {
+    set_error(LAP_ERR_ARG, "lap_dot_noalloc: device %d is beyond the %d supported", dev,
+              kNoAllocDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(LAP_ERR_HIP, "lap_dot_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[dev] = static_cast<double *>(ptr);
+  }
+  return g_scratch_dev[dev];
+}
+
 }  // namespace lap
 
 using namespace lap;
 
 extern "C" {
+
+int lap_dot_noalloc(const int n, const double *v, const double *w, double *result) {
+  if (n < 0) {
+    set_error(LAP_ERR_ARG, "lap_dot_noalloc: n = %d is negative", n);
+    return LAP_ERR_ARG;
+  }
+  const size_t bytes = sizeof(double) * (size_t)n;
+  if (!check_device("lap_dot_noalloc", {{result, "result", sizeof(double)}}) ||
+      (n > 0 && !check_device("lap_dot_noalloc", {{v, "v", bytes}, {w, "w", bytes}})))
+    return LAP_ERR_ARG;
+  double *partials = dot_scratch();
+  if (!partials) return lap_last_status();
+  // hip_dot's kernels and grid, so the result has hip_dot's bits
+  const unsigned g = sgrid(n);
+  if (g > (unsigned)kNoAllocPartials) {
+    set_error(LAP_ERR_ARG, "lap_dot_noalloc: %u partial sums exceed the %d of its scratch", g,
+              kNoAllocPartials);
+    return LAP_ERR_ARG;
+  }
+  k_dot_partial<kBlock><<<g, kBlock, 0, stream()>>>(n, v, w, partials);
+  k_dot_final<kBlock><<<1, kBlock, 0, stream()>>>((int)g, partials, result);
+  if (!LAP_HIP(hipGetLastError()) || !LAP_HIP(hipStreamSynchronize(stream())))
+    return LAP_ERR_HIP;
+  return LAP_OK;
+}
 
 double hip_dot(const int n, const double *v, const double *w) {
   if (n <= 0) return 0.0;

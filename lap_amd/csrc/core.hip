@@ -236,6 +236,7 @@ void lap_release_all(void) {
 void initialize_handles(void) {
   stream();
   reduce_ws();
+  dot_scratch();  // lap_dot_noalloc's scratch on the current device
 }
 
 void *mallocForDevice(void *x, int n, int size) {

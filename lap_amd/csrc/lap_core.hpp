@@ -100,6 +100,9 @@ struct ReduceWs {
   double *scalars;   // [64] general device scalars
 };
 ReduceWs &reduce_ws();
+// The current device's address of lap_dot_noalloc's static scratch array
+// (looked up once per device; null and an error recorded on failure).
+double *dot_scratch();
 
 // -------------------------------------------------------- device helpers
 __device__ __forceinline__ double wave_sum(double v) {

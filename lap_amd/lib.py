@@ -50,6 +50,7 @@ _SIGS = {
     "initialize_and_analyze_L_and_U_solve": (None, [C.c_int, C.c_int, PI, PI, PD, C.c_int, PI,
                                                     PI, PD]),
     "hip_dot": (C.c_double, [C.c_int, PD, PD]),
+    "lap_dot_noalloc": (C.c_int, [C.c_int, PD, PD, PD]),
     "hip_scal": (None, [C.c_int, C.c_double, PD]),
     "hip_axpy": (None, [C.c_int, C.c_double, PD, PD]),
     "hip_csr_matvec": (None, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.POINTER(C.c_double),
