@@ -143,6 +143,7 @@ void orc_ichol_factor(int n, const int *uia, const int *uja, double *ua,
     }
   }
   /* L = U^T: walking U by rows fills each L row in ascending column order */
+  if (n <= 0) return;
   int *fill = (int *)calloc((size_t)n, sizeof(int));
   for (int i = 0; i < n; ++i)
     for (int e = uia[i]; e < uia[i + 1]; ++e) {

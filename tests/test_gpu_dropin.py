@@ -97,3 +97,40 @@ def test_reference_laplacian_driver_on_gfx950_backend(tmp_path, graph, kind, io)
           f"cpu it {ci} res {cres:.6e} ({creason})", flush=True)
     assert greason == creason == "CG converged"
     assert abs(gi - ci) <= 1
+
+
+DROPIN_SPMV = os.path.join(REF, "lap_dropin_spmv")
+CPU_SPMV = os.path.join(REF, "lap_cpu_spmv")
+
+
+def run_spmv(exe, mm, trials):
+    out = subprocess.run([exe, mm, str(trials)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    norm = float(re.search(r"Norm of A\*x\s*:\s*(\S+)", out.stdout).group(1))
+    per = float(re.search(r"Time \(av\., per one\)\s*:\s*(\S+)", out.stdout).group(1))
+    return norm, per
+
+
+@pytest.mark.skipif(not (os.path.exists(DROPIN_SPMV) and os.path.exists(CPU_SPMV)),
+                    reason="SpMV drop-in binary not built (needs the reference sources)")
+@pytest.mark.parametrize("shift", [0.0, 1e-8])
+@pytest.mark.parametrize("dims", [[128, 128], [40, 30, 20]])
+def test_reference_spmv_driver_on_gfx950_backend(tmp_path, dims, shift):
+    """mm_driver.c (the reference's SpMV driver) linked against the library:
+    y = A*1 through analyze_spmv + csr_matvec (mm_driver.c:94-117), then
+    ||y|| through dot (:128).  Every row of an unshifted graph Laplacian sums
+    to exactly 0 in CSR order (integer partial sums), so ||A*1|| is exactly
+    0, as on the CPU binary; with a 1e-8 shift each row sums to ~1e-8 and
+    the norm matches the CPU binary's within the dot's summation order."""
+    print(f"[case] dropin spmv dims={dims} shift={shift:g}", flush=True)
+    A = grid_laplacian(dims, shift)
+    mm = str(tmp_path / "A.mtx")
+    write_mm(mm, A)
+    gnorm, gper = run_spmv(DROPIN_SPMV, mm, 10)
+    cnorm, _ = run_spmv(CPU_SPMV, mm, 10)
+    print(f"[diag] dropin spmv {dims} shift {shift:g}: gpu ||A*1|| {gnorm:.16e} "
+          f"({gper * 1e6:.1f} us per SpMV incl. sync), cpu {cnorm:.16e}", flush=True)
+    if shift == 0.0:
+        assert gnorm == 0.0 and cnorm == 0.0
+    else:
+        assert cnorm > 0.0 and abs(gnorm - cnorm) <= 1e-12 * cnorm
