@@ -65,16 +65,44 @@ def measured_traffic(size: int):
     return None
 
 
+def host_cores() -> int:
+    """CPUs this process may run on (its affinity mask), not the machine's."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # not Linux
+        return os.cpu_count() or 1
+
+
+def host_cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(Ah, b, prec: str, tol: float, gpu_iters: int, sample_iters: int):
     """The reference's OpenMP cg() (compiled from its sources into
     oracle/_ref/) timed on this host for `sample_iters` iterations of the same
-    system; seconds to the GPU's iteration count are extrapolated from the
-    per-iteration time.  Falls back to the oracle's OpenMP restatement."""
+    system, inside the bracket cg_driver2.c:392-408 puts around cg(); seconds
+    to the GPU's iteration count are extrapolated from the per-iteration time
+    (a full solve at ~60 ms per iteration would take ~95 s).  The build is the
+    reference's as written: SpMV on its hard-coded 16 threads
+    (openmp_blas.c:37), dot on OMP_NUM_THREADS, other vector ops serial, and
+    the SpMV's shared loop variable `col` raced on as in openmp_blas.c:35-51.
+    Beside it, the all-cores figure: the oracle's OpenMP restatement with the
+    SpMV and dot on every core this process may use.  Falls back to the
+    oracle for the first figure when the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    cores = host_cores()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cores
     maxit = max(sample_iters - 1, 1)
+    orc = oracle.Oracle()
     if os.path.exists(oracle.REF_OMP):
         ref = oracle.Reference(oracle.REF_OMP)
         pdk = ref.make_pdata(Ah, prec)
@@ -83,23 +111,89 @@ def cpu_baseline(Ah, b, prec: str, tol: float, gpu_iters: int, sample_iters: int
         dt = time.perf_counter() - t0
         kind = "reference"
         what = ("the reference's OpenMP cg() (openmp_blas.c build: SpMV on its hard-coded 16 "
-                f"threads, dot on OMP_NUM_THREADS={threads}, other vector ops serial)")
+                f"threads with the racy shared `col`, dot on OMP_NUM_THREADS={threads}, other "
+                "vector ops serial)")
     else:
-        orc = oracle.Oracle()
         t0 = time.perf_counter()
         _, it, _, _ = orc.cg(Ah, b, prec, tol=tol, maxit=maxit, omp=(16, threads))
         dt = time.perf_counter() - t0
         kind = "port"
         what = "the oracle's OpenMP restatement of the reference CPU path"
     per_iter = dt / it
+    prec_h = orc.make_prec(Ah, prec)
+    t0 = time.perf_counter()
+    _, it_all, _, _ = orc.cg(Ah, b, prec, tol=tol, maxit=maxit, prec=prec_h, omp=(cores, cores))
+    dt_all = time.perf_counter() - t0
+    per_all = dt_all / it_all
     return {
         "value": per_iter * gpu_iters,
         "unit": "s",
-        "cores": max(16, threads),
+        "cores": min(cores, max(16, threads)),
         "kind": kind,
         "sample": (f"{it} PCG iterations of {what} on the same 256^3 system "
                    f"({dt:.2f} s, {per_iter * 1e3:.1f} ms/iteration), scaled to the "
                    f"GPU's {gpu_iters} iterations"),
+        "host_cores": cores,
+        "host_cpu": host_cpu_model(),
+        "all_cores": {
+            "value": per_all * gpu_iters, "unit": "s", "cores": cores, "kind": "port",
+            "sample": (f"{it_all} PCG iterations of the oracle's OpenMP restatement with SpMV "
+                       f"and dot on all {cores} cores ({dt_all:.2f} s, "
+                       f"{per_all * 1e3:.1f} ms/iteration), scaled to {gpu_iters} iterations"),
+        },
+    }
+
+
+def bench_c1(S, torch, np, reps: int = 3) -> dict:
+    """The reference's own CPU-runnable case: 2D 5-point Laplacian 128x128 +
+    1e-8*I, b ~ U(-1,1) seed 12345 mean-subtracted, Jacobi PCG to rel-res 1e-8.
+    The GPU solve (median of `reps`, inputs resident) beside the reference's
+    serial simple_blas cg() on one core, timed around cg() as
+    cg_driver2.c:392-408 does (median of `reps`; the oracle's serial
+    restatement when the reference build is absent)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from lap_amd.problems import grid_laplacian, rhs_uniform
+
+    Ah = grid_laplacian([128, 128], 1e-8)
+    b_h = rhs_uniform(Ah.n, 12345)
+    A = S.DeviceCSR.from_csr(Ah)
+    b = S.to_dev(b_h)
+    P = S.make_preconditioner(A, Ah, "line_jacobi")
+    S.pcg(A, b, P, tol=1e-8, maxit=25000)
+    walls, r = [], None
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = S.pcg(A, b, P, tol=1e-8, maxit=25000)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    cpu, it_c = [], None
+    if oracle.have_ref():
+        ref = oracle.Reference(oracle.REF_CPU)
+        pdk = ref.make_pdata(Ah, "line_jacobi")
+        kind = "reference"
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, it_c, _, _ = ref.cg(Ah, b_h, "line_jacobi", tol=1e-8, maxit=25000, pdk=pdk)
+            cpu.append(time.perf_counter() - t0)
+    else:
+        orc = oracle.Oracle()
+        prec = orc.make_prec(Ah, "line_jacobi")
+        kind = "port"
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, it_c, _, _ = orc.cg(Ah, b_h, "line_jacobi", tol=1e-8, maxit=25000, prec=prec)
+            cpu.append(time.perf_counter() - t0)
+    gpu_s, cpu_s = float(np.median(walls)), float(np.median(cpu))
+    return {
+        "workload": "2D 5-point graph Laplacian 128x128 + 1e-8*I, b~U(-1,1) seed 12345 "
+                    "mean-subtracted, Jacobi PCG to rel-res 1e-8",
+        "gpu": {"value": gpu_s, "unit": "s", "iterations": r.iterations, "flag": r.flag,
+                "device_ms": r.gpu_ms},
+        "cpu_serial": {"value": cpu_s, "unit": "s", "iterations": it_c, "cores": 1,
+                       "kind": kind, "sample": f"full solve, median of {reps}"},
+        "speedup": cpu_s / gpu_s,
     }
 
 
@@ -627,12 +721,17 @@ def cpu_input_baseline(np, size: int = 128) -> dict:
         shutil.rmtree(d, ignore_errors=True)
 
 
-def pmc_probe(S, torch, size: int, reps: int) -> int:
+def pmc_probe(S, torch, size: int, reps: int, walk: int = 0, bound: bool = False) -> int:
     """Launches for rocprofv3 --pmc: 4 elementwise copies (dst = src * 1) of a
     1 GiB float64 buffer (read 1 GiB, write 1 GiB each: the calibration of
     FETCH_SIZE and WRITE_SIZE) and `reps` launches of the fused SpMV on the
-    size^3 Laplacian."""
-    from lap_amd.problems import grid_laplacian
+    size^3 Laplacian, its row blocks walked in order `walk` (option
+    spmv_walk).  `bound`: the same arrays with every column index replaced by
+    its own row (same bytes and the same LDS two-phase structure, but each
+    row block reads only its own slice of x, once), which bounds what any
+    x-reuse scheme could gain."""
+    import numpy as np
+    from lap_amd.problems import CSR, grid_laplacian
     src = torch.randn(1 << 27, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(4):
@@ -641,12 +740,19 @@ def pmc_probe(S, torch, size: int, reps: int) -> int:
     log("copy calibration: 4 x 1 GiB")
     del src, dst
     Ah = grid_laplacian([size] * 3, 1e-8)
-    A = S.DeviceCSR.from_csr(Ah)
-    x = torch.randn(A.n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    ms = S.spmv_timed(A, x, y, reps, fused=True)
-    log(f"fused SpMV x{reps}: {ms:.4f} ms per launch, algorithmic bytes "
-        f"{spmv_bytes(A.n, A.nnz)}")
+    if bound:
+        rows = np.repeat(np.arange(Ah.n, dtype=np.int32), np.diff(Ah.ia))
+        Ah = CSR(Ah.n, Ah.ia, rows, Ah.a)
+    prev = S.set_option("spmv_walk", walk)
+    try:
+        A = S.DeviceCSR.from_csr(Ah)
+        x = torch.randn(A.n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        ms = S.spmv_timed(A, x, y, reps, fused=True)
+    finally:
+        S.set_option("spmv_walk", prev)
+    log(f"fused SpMV x{reps} (spmv_walk {walk}{', ja = row' if bound else ''}): {ms:.4f} ms "
+        f"per launch, algorithmic bytes {spmv_bytes(A.n, A.nnz)}")
     return 0
 
 
@@ -658,7 +764,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--prec", default="line_jacobi")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true")
@@ -676,6 +782,10 @@ def main() -> int:
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the kernels the HBM-traffic counters are read on: a 1 GiB "
                          "device copy (calibration) and the fused SpMV, then exit")
+    ap.add_argument("--pmc-walk", type=int, default=0,
+                    help="--pmc-probe: the SpMV's row-block walk (option spmv_walk)")
+    ap.add_argument("--pmc-bound", action="store_true",
+                    help="--pmc-probe: every column index replaced by its row (x read once)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -690,7 +800,7 @@ def main() -> int:
 
     torch.cuda.set_device(0)
     if args.pmc_probe:
-        return pmc_probe(S, torch, args.size, args.spmv_reps)
+        return pmc_probe(S, torch, args.size, args.spmv_reps, args.pmc_walk, args.pmc_bound)
     t0 = time.perf_counter()
     Ah = grid_laplacian([args.size] * 3, 1e-8)
     b_h = rhs_normal(Ah.n, 12345)
@@ -810,7 +920,8 @@ def main() -> int:
                          ("block", lambda: bench_block(S, torch, np)),
                          ("batched", lambda: bench_batched(S, torch, np)),
                          ("adjoint", lambda: bench_adjoint(torch, np)),
-                         ("input", lambda: bench_input(S, torch, np, A, Ah, args.size))):
+                         ("input", lambda: bench_input(S, torch, np, A, Ah, args.size)),
+                         ("c1", lambda: bench_c1(S, torch, np))):
             if args.only and name not in args.only.split(","):
                 continue
             t0 = time.perf_counter()
