@@ -32,6 +32,7 @@ bool hip_ok(hipError_t e, const char *what, const char *file, int line);
 
 // ------------------------------------------------------------------ runtime
 hipStream_t stream();
+int host_threads();  // host threads the library's parallel host code uses
 int device_cus();
 void sync();  // hipStreamSynchronize(stream()) with error capture
 // True when p is device (or managed) memory of the HIP runtime.  Used to

@@ -54,7 +54,9 @@
 // bounded and raises the shared error flag when it runs out, so the kernel
 // always ends.  Same row arithmetic as the level launches: bitwise identical.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
+#include <thread>
 
 #include "lap_core.hpp"
 
@@ -905,17 +907,31 @@ static void tri_chunks_build(TriPlan &p, const std::vector<int> &ia, const std::
         }
     }
   }
-  ChunkSched best, cur;
-  size_t best_i = 0;
-  for (size_t ci = 0; ci < cands.size(); ++ci) {
-    if (!sched_build(cur, cands[ci].first, cands[ci].second, ia, ja, level, order, kChStepRows))
-      continue;
-    if (best.cost < 0 || cur.cost < best.cost) {
-      std::swap(best, cur);
-      best_i = ci;
-    }
+  // the candidates are independent: one host thread each (the lowest cost
+  // wins, ties to the earlier candidate, so the choice does not depend on
+  // thread timing)
+  std::vector<ChunkSched> built(cands.size());
+  {
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+      for (size_t ci; (ci = next.fetch_add(1)) < cands.size();)
+        if (!sched_build(built[ci], cands[ci].first, cands[ci].second, ia, ja, level, order,
+                         kChStepRows))
+          built[ci] = ChunkSched();
+    };
+    const int T = std::max(1, std::min(host_threads(), (int)cands.size()));
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(work);
+    work();
+    for (std::thread &t : th) t.join();
   }
-  if (best.cost < 0) return;
+  size_t best_i = 0;
+  for (size_t ci = 0; ci < cands.size(); ++ci)
+    if (built[ci].cost >= 0 && (built[best_i].cost < 0 || built[ci].cost < built[best_i].cost))
+      best_i = ci;
+  if (cands.empty() || built[best_i].cost < 0) return;
+  ChunkSched best = std::move(built[best_i]);
+  built.clear();
   if (getenv("LAP_SCHED_LOG"))
     fprintf(stderr, "[lapgfx] %s triangle n=%d: chunked schedule %zu of %zu, %d chunks, %zu steps, "
             "%zu waits, simulated length %lld (one launch per level: %d)\n",
