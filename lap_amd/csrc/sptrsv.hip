@@ -667,12 +667,19 @@ __global__ __launch_bounds__(kBlock + kWave) void k_tri_chunks(
 
 // xc[t] = x[row of schedule position t]: the right-hand side in the chunked
 // schedule's order, for k_tri_chunks' prefetch.
+// The rows of neighbouring schedule positions share lines of x, so each XCD
+// (workgroups are dealt to the 8 XCDs round-robin) sweeps one contiguous
+// eighth of the positions: its workgroups then read the same lines out of
+// their own L2.  The grid is a multiple of 8.
 __global__ __launch_bounds__(kBlock) void k_ch_gather(int n, const int *__restrict__ crow,
                                                       const double *__restrict__ x,
                                                       double *__restrict__ xc,
                                                       const int *__restrict__ done) {
   if (done && *done) return;
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock)
+  const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+  const long long lo = (long long)n * xcd / 8, hi = (long long)n * (xcd + 1) / 8;
+  for (long long t = lo + (long long)(blockIdx.x >> 3) * kBlock + threadIdx.x; t < hi;
+       t += (long long)per * kBlock)
     xc[t] = x[crow[t]];
 }
 
@@ -1170,7 +1177,7 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
       const TriPlan::Chunks &ch = p.ch;
       k_ch_reset<<<(ch.nchunks + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(ch.nchunks, prog,
                                                                              done);
-      k_ch_gather<<<grid_for((size_t)p.n, 4, kBlock, 4096), kBlock, 0, stream()>>>(
+      k_ch_gather<<<2048, kBlock, 0, stream()>>>(
           p.n, ch.crow->as<int>(), x, ell->xc->as<double>(), done);
 #define LAP_CH_ARGS                                                                            \
   ch.cstep->as<int>(), ch.sptr->as<int>(), ch.slev->as<int>(), ch.wptr->as<int>(),             \
