@@ -33,22 +33,10 @@ def spmv_bytes(n: int, nnz: int) -> int:
     return 12 * nnz + 4 * (n + 1) + 16 * n
 
 
-def jacobi_iteration_bytes(n: int, nnz: int, variant: int = 0) -> int:
-    """Bytes of one Jacobi PCG iteration.  Variant 0: SpMV + 88 B per row
-    (K_B reads x, r, p, q, 1/diag and writes x, r; K_C reads p, r, 1/diag and
-    writes p).  Variant 1 (x updated in K_C, where p is read anyway): SpMV +
-    80 B per row.  Variant 2 (p update in the SpMV gather): 12 nnz + 4(n+1) +
-    88 B per row."""
-    if variant == 2:
-        return 12 * nnz + 4 * (n + 1) + 88 * n
-    return spmv_bytes(n, nnz) + (80 if variant == 1 else 88) * n
-
-
-def fused_kernel_bytes(n: int, nnz: int) -> int:
-    """k_jac_fused: the CSR (12 B per nonzero, 4 B per row pointer), the
-    gathered p_old, r and 1/diag (24 B per row), p_new and q written (16 B),
-    x read and written (16 B)."""
-    return 12 * nnz + 4 * (n + 1) + 56 * n
+def jacobi_iteration_bytes(n: int, nnz: int) -> int:
+    """Bytes of one Jacobi PCG iteration: the SpMV + 80 B per row (K_B reads
+    r, q, 1/diag and writes r; K_C reads x, p, r, 1/diag and writes x, p)."""
+    return spmv_bytes(n, nnz) + 80 * n
 
 
 def measured_traffic(size: int):
@@ -768,9 +756,6 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--jacobi-variant", type=int, default=1,
-                    help="Jacobi iteration: 0 three launches, x beside r; 1 three launches, "
-                         "x beside p; 2 two launches, p update in the SpMV gather")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the preconditioner, multi-RHS, batched, adjoint and input-path "
                          "measurements")
@@ -808,9 +793,7 @@ def main() -> int:
     A = S.DeviceCSR.from_csr(Ah)
     b = S.to_dev(b_h)
     P = S.make_preconditioner(A, Ah, args.prec)
-    S.set_option("jacobi_variant", args.jacobi_variant)
-    fused = args.jacobi_variant == 2 and args.prec == "line_jacobi"
-    log(f"setup done (Jacobi iteration variant {args.jacobi_variant})")
+    log("setup done")
 
     for w in range(args.warmup):
         r = S.pcg(A, b, P, tol=args.tol, maxit=25000)
@@ -830,19 +813,13 @@ def main() -> int:
 
     # dominant kernel: the fused SpMV q = A p (+ p.q), timed live with HIP
     # events on the library stream
-    if fused:
-        spmv_ms = S.fused_timed(A, P.keep["d_r"], args.spmv_reps)
-        bytes_spmv = fused_kernel_bytes(A.n, A.nnz)
-        kname = ("k_jac_fused (p = beta p + d_r r in the CSR SpMV gather, q = A p with p.q, "
-                 "x += alpha p)")
-    else:
-        x = torch.randn(A.n, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        spmv_ms = S.spmv_timed(A, x, y, args.spmv_reps, fused=True)
-        bytes_spmv = spmv_bytes(A.n, A.nnz)
-        kname = "k_spmv_pq (fused CSR SpMV q=A p with p.q)"
+    x = torch.randn(A.n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    spmv_ms = S.spmv_timed(A, x, y, args.spmv_reps, fused=True)
+    bytes_spmv = spmv_bytes(A.n, A.nnz)
+    kname = "k_spmv_pq (fused CSR SpMV q=A p with p.q)"
     achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9
-    it_bytes = jacobi_iteration_bytes(A.n, A.nnz, args.jacobi_variant if args.prec == "line_jacobi" else 0)
+    it_bytes = jacobi_iteration_bytes(A.n, A.nnz)
     log(f"dominant kernel {spmv_ms:.4f} ms -> {achieved:.0f} GB/s")
 
     out = {
@@ -863,7 +840,6 @@ def main() -> int:
                          f"12345 mean-subtracted, FP64 PCG ({args.prec}) from x0=0 to "
                          f"rel-res {args.tol:g}"),
             "n": A.n, "nnz": A.nnz, "preconditioner": args.prec,
-            "jacobi_variant": args.jacobi_variant,
             "iterations": its, "flag": sorted(flags),
         },
         "device_ms_per_solve": gpu_ms,
@@ -875,7 +851,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None if fused else measured_traffic(args.size),
+            "traffic": measured_traffic(args.size),
             "algorithmic_bytes": bytes_spmv,
             "avg_ms": spmv_ms,
         },

@@ -319,28 +319,20 @@ double lap_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double
 double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
                           const double *p, double *q, int reps);
 
-/* Same for the two-launch Jacobi iteration's first kernel (jacobi_variant 2):
- * p_new = beta p_old + d_r r in the SpMV gather, q = A p_new with p.q, and
- * x += alpha p_old; timed with beta = 0 and alpha = 0.5. */
-double lap_pcg_fused_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
-                           const double *p_old, double *p_new, double *q, const double *r,
-                           const double *d_r, double *x, int reps);
-
 /* Library configuration knobs (for tuning sweeps): name/value pairs.
  * "spmv_rows" (rows per SpMV block), "spmv_walk" (row-block order of the
- * persistent SpMV grid: 0 cyclic, 1 per-XCD eighths, 2 contiguous chunk per
- * workgroup), "spmv_variant" (SpMV body, 0-6), "vec_grid" (workgroups of
- * the vector kernels), "graph_iters" (iterations per HIP graph launch),
+ * persistent SpMV grid: 0 cyclic, 3 groups of four consecutive blocks per
+ * XCD), "spmv_wg_per_cu" (its workgroups per CU), "vec_grid" (workgroups
+ * of the vector kernels), "graph_iters" (iterations per HIP graph launch),
  * "use_graph", "spmm_wg_per_cu" (block CG SpMM workgroups per CU),
- * "sptrsv_mode" (triangular solves: 0 one launch per level, 1 one
- * sync-free launch, 2 one launch with a grid barrier per level, 3, the
+ * "sptrsv_mode" (triangular solves: 0 one launch per level, 3, the
  * default, one persistent launch over chunks of rows with point-to-point
- * waits), "jacobi_variant" (the Jacobi PCG iteration: 0 three launches
- * with x updated beside r; 1, the default, three launches with x updated
- * beside p, which reads p once less; 2 two launches, the p update computed
- * in the next SpMV's gather).  Results do not depend on any of them except
- * vec_grid, which sets the dot-product summation order.  Returns the previous value,
- * or -1 for an unknown name. */
+ * waits), "sptrsv_long" (1: rows of more than 32 entries summed as a
+ * workgroup tree), "fuse_sweeps", "spmv_format" (0 CSR, 1 value-indexed,
+ * 2 delta-indexed), "mixed_vectors" (lap_pcg_mixed inner vectors: 0 FP32,
+ * 1 FP64).  Results do not depend on them except vec_grid, spmv_walk and
+ * spmv_wg_per_cu (the dot-product summation order), sptrsv_long and
+ * mixed_vectors.  Returns the previous value, or -1 for an unknown name. */
 int lap_set_option(const char *name, int value);
 
 /* No counterpart in the reference: hip_dot for callers that cannot allocate

@@ -158,10 +158,8 @@ int lap_set_option(const char *name, int value) {
   if (!strcmp(name, "spmv_rows")) slot = &o.spmv_rows;
   else if (!strcmp(name, "spmv_walk")) slot = &o.spmv_walk;
   else if (!strcmp(name, "vec_grid")) slot = &o.vec_grid;
-  else if (!strcmp(name, "spmv_variant")) slot = &o.spmv_variant;
   else if (!strcmp(name, "graph_iters")) slot = &o.graph_iters;
   else if (!strcmp(name, "use_graph")) slot = &o.use_graph;
-  else if (!strcmp(name, "jacobi_variant")) slot = &o.jacobi_variant;
   else if (!strcmp(name, "fuse_sweeps")) slot = &o.fuse_sweeps;
   else if (!strcmp(name, "spmv_format")) slot = &o.spmv_format;
   else if (!strcmp(name, "mixed_vectors")) slot = &o.mixed_vectors;
@@ -169,16 +167,9 @@ int lap_set_option(const char *name, int value) {
   else if (!strcmp(name, "sptrsv_mode")) slot = &o.sptrsv_mode;
   else if (!strcmp(name, "spmm_wg_per_cu")) slot = &o.spmm_wg_per_cu;
   else if (!strcmp(name, "spmv_wg_per_cu")) slot = &o.spmv_wg_per_cu;
-  else if (!strcmp(name, "sf_wg_per_cu")) slot = &o.sf_wg_per_cu;
-  else if (!strcmp(name, "sf_naps")) slot = &o.sf_naps;
   if (!slot) return -1;
-  if ((!strcmp(name, "spmv_walk") && (value < 0 || value > 5)) ||
+  if ((!strcmp(name, "spmv_walk") && value != 0 && value != 3) ||
       (!strcmp(name, "vec_grid") && (value < 1 || value > kMaxPartialBlocks / 2))) {
-    set_error(LAP_ERR_ARG, "option %s: value %d out of range", name, value);
-    return *slot;
-  }
-  if ((!strcmp(name, "sf_wg_per_cu") && (value < 1 || value > 16)) ||
-      (!strcmp(name, "sf_naps") && (value < 1 || value > 64))) {
     set_error(LAP_ERR_ARG, "option %s: value %d out of range", name, value);
     return *slot;
   }
@@ -190,8 +181,8 @@ int lap_set_option(const char *name, int value) {
     set_error(LAP_ERR_ARG, "spmm_wg_per_cu must be in [1, 16]");
     return *slot;
   }
-  if (!strcmp(name, "sptrsv_mode") && (value < 0 || value > 3)) {
-    set_error(LAP_ERR_ARG, "sptrsv_mode must be 0, 1, 2 or 3");
+  if (!strcmp(name, "sptrsv_mode") && value != 0 && value != 3) {
+    set_error(LAP_ERR_ARG, "sptrsv_mode must be 0 or 3");
     return *slot;
   }
   if (!strcmp(name, "sptrsv_long") && (value < 0 || value > 1)) {
@@ -208,14 +199,6 @@ int lap_set_option(const char *name, int value) {
   }
   if (!strcmp(name, "fuse_sweeps") && (value < 0 || value > 1)) {
     set_error(LAP_ERR_ARG, "fuse_sweeps must be 0 or 1");
-    return *slot;
-  }
-  if (!strcmp(name, "jacobi_variant") && (value < 0 || value > 2)) {
-    set_error(LAP_ERR_ARG, "jacobi_variant must be 0, 1 or 2");
-    return *slot;
-  }
-  if (!strcmp(name, "spmv_variant") && (value < 0 || value > 6)) {
-    set_error(LAP_ERR_ARG, "spmv_variant must be in [0, 6]");
     return *slot;
   }
   if (!strcmp(name, "spmv_rows") && (value < 1 || value > 2 * kBlock)) {

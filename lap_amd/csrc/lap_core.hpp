@@ -189,8 +189,8 @@ __device__ __forceinline__ double sum_partials(const double *partials, int cnt, 
 // skipping block(s) >= nblk (only the last group can overrun).
 struct BlockWalk {
   int begin, end, step;
-  int nblk;     // row blocks of the plan (the nonzero offsets follow rowblk)
-  int g = 0;    // group size of modes 3-5 (0: s is the block)
+  int nblk;     // row blocks of the plan
+  int g = 0;    // group size of walk 3 (0: s is the block)
   int gk = 0;   // first block of this XCD's first group
   __device__ __forceinline__ int block(int s) const {
     return g ? (s / g) * (8 * g) + gk + (s % g) : s;
@@ -198,8 +198,8 @@ struct BlockWalk {
 };
 __device__ __forceinline__ BlockWalk block_walk(int nblk, int mode) {
   const int g = gridDim.x, id = blockIdx.x;
-  if (mode >= 3 && mode <= 5 && g >= 8) {
-    const int gs = mode == 3 ? 4 : mode == 4 ? 8 : 2;
+  if (mode == 3 && g >= 8) {
+    const int gs = 4;
     const int k = id % 8, lw = id / 8;
     const int nw = g / 8 + (k < g % 8 ? 1 : 0);
     const int base = gs * k;
@@ -209,36 +209,22 @@ __device__ __forceinline__ BlockWalk block_walk(int nblk, int mode) {
     w.gk = base;
     return w;
   }
-  if (mode == 2) {
-    const int chunk = (nblk + g - 1) / g;
-    const long long b0 = (long long)id * chunk;
-    const int begin = b0 < nblk ? (int)b0 : nblk;
-    const int end = begin + chunk < nblk ? begin + chunk : nblk;
-    return BlockWalk{begin, end, 1, nblk};
-  }
-  if (mode != 1 || g < 8) return BlockWalk{id, nblk, g, nblk};
-  const int k = id % 8, lw = id / 8;
-  const int nw = g / 8 + (k < g % 8 ? 1 : 0);
-  const int lo = (int)(((long long)nblk * k) / 8), hi = (int)(((long long)nblk * (k + 1)) / 8);
-  return BlockWalk{lo + lw, hi, nw, nblk};
+  return BlockWalk{id, nblk, g, nblk};
 }
 
 // -------------------------------------------------------------- options
 struct Options {
   int spmv_rows = 256;   // rows per SpMV block (<= kBlock * 2)
-  int spmv_walk = 0;     // row-block walk of the persistent SpMV grid (0..5)
+  int spmv_walk = 0;     // row-block walk of the persistent SpMV grid: 0 cyclic,
+                         // 3 groups of 4 consecutive blocks per XCD
   int vec_grid = 768;    // workgroups of the streaming vector kernels (3 per CU)
   int spmv_wg_per_cu = 8;  // persistent SpMV grid: workgroups per CU
-  int spmv_variant = 0;  // SpMV body variant (spmv_core.hpp)
   int graph_iters = 16;  // PCG iterations per graph launch
   int use_graph = 1;
   int spmm_wg_per_cu = 4;  // block CG SpMM grid: workgroups per CU
-  int sptrsv_mode = 3;     // triangular solves: 0 one launch per level,
-                           // 1 one sync-free launch, 2 one launch with a
-                           // grid barrier per level, 3 one persistent launch
-                           // over chunks of rows with point-to-point waits
-  int sf_wg_per_cu = 8;    // sync-free solve: workgroups per CU
-  int sf_naps = 1;         // sync-free solve: sleeps (~512 cycles) between polls
+  int sptrsv_mode = 3;     // triangular solves: 0 one launch per level, 3 one
+                           // persistent launch over chunks of rows with
+                           // point-to-point waits
   int fuse_sweeps = 1;     // it_jacobi / GS_it / GS_it2: one launch per sweep
   int mixed_vectors = 0;   // lap_pcg_mixed inner vectors: 0 FP32, 1 FP64
   int sptrsv_long = 0;     // level solves' rows > kTriLongRow entries: 0 summed
@@ -247,9 +233,6 @@ struct Options {
                            // matrix has at most 255 distinct values, 2 also
                            // the columns as offset indices (<= 255 distinct
                            // col - row, rows <= 64 entries), else as 1
-  int jacobi_variant = 1;  // Jacobi PCG iteration: 0 three kernels, x in K_B;
-                           // 1 three kernels, x in K_C; 2 two kernels (p in the
-                           // SpMV gather)
 };
 Options &options();
 
@@ -273,8 +256,7 @@ void forget_plans(const void *p);
 struct SpmvPlan {
   int n = 0, nnz = 0;
   int nblocks = 0;        // row blocks
-  DevBufPtr rowblk;       // int[2 * (nblocks + 1)]: first row of each block, then
-                          // its first nonzero (ia at that row)
+  DevBufPtr rowblk;       // int[nblocks + 1]: first row of each block
   int max_row = 0;
 };
 // Finds or builds the plan for (ia, ja, n, nnz) (device arrays).
@@ -327,7 +309,6 @@ struct TriPlan {
   bool lower = true;
   int nlevels = 0;
   std::vector<int> level_ptr;  // host, nlevels + 1
-  DevBufPtr level_ptr_dev;     // the same on the device (level-barrier mode)
   DevBufPtr desc;              // int4[n] in level order: {row, ia[row], ia[row+1], 0}
   int max_row = 0;             // longest row (entries, pivot included)
   DevBufPtr order;             // int[n]: rows sorted by level
@@ -337,8 +318,6 @@ struct TriPlan {
   // level_ptr[l + 1]).  A long row is solved by a whole workgroup.
   std::vector<int> level_long;  // host, nlevels
   bool has_diag = true;        // every row holds its pivot
-  int nchunks = 0;             // sync-free mode: <= 64 rows of one level each
-  DevBufPtr chunk_ptr;         // int[nchunks + 1], offsets into order
   // Chunked schedule (one persistent workgroup per chunk of consecutive rows
   // in solve order; built when every row holds at most 8 entries).  Chunk c
   // runs its rows level by level ("steps"); before a step it waits until

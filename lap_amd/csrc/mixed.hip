@@ -116,7 +116,7 @@ __global__ __launch_bounds__(BS) void k_mx_init(int nblk, const int *__restrict_
   __shared__ double prod[TILE];
   __shared__ double red[BS / kWave];
   Resid64Epi<VT> epi{rhs, r32, d_r, p32, 0.0, 0.0};
-  spmv_walk<BS, TILE, 0>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
+  spmv_walk<BS, TILE>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
   const double t_rz = block_sum<BS>(epi.rz, red);
   const double t_rr = block_sum<BS>(epi.rr, red);
   if (threadIdx.x == 0) {
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(BS) void k_mx_resid(int nblk, const int *__restrict
   const double rr_rec = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
   if (!need_refine(rr_rec, ctl->iter + 1, ctl)) return;
   Resid64Epi<VT> epi{rhs, r32, d_r, nullptr, 0.0, 0.0};
-  spmv_walk<BS, TILE, 0>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
+  spmv_walk<BS, TILE>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
   const double t_rz = block_sum<BS>(epi.rz, red);
   const double t_rr = block_sum<BS>(epi.rr, red);
   if (threadIdx.x == 0) {

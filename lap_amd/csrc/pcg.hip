@@ -35,7 +35,7 @@ struct PcgCtl {
   double rho;       // r_k.w_k of the iteration in flight (alpha = rho / p.q)
   double rho_prev;  // generic path: rho of the previous iteration
   double tolrel, res0, res, pend_rho;
-  double alpha_x;   // fused Jacobi: alpha of the x update still to apply
+  double alpha_x;   // Jacobi: alpha of the x update still to apply
   int iter;         // finished iterations, as seen by the kernels
   int maxit, done, flag;
   int pend_iter, pend_valid, last_iter, pend_done;
@@ -86,7 +86,7 @@ struct InitEpi {
 };
 
 // r = b - A x0; partials of r.r
-template <int BS, int TILE, int V>
+template <int BS, int TILE>
 __global__ __launch_bounds__(BS) void k_init(int nblk, const int *__restrict__ rowblk,
                                              const int *__restrict__ ia,
                                              const int *__restrict__ ja,
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(BS) void k_init(int nblk, const int *__restrict__ r
   __shared__ double prod[TILE];
   __shared__ double red[BS / kWave];
   InitEpi epi{r, rhs, 0.0};
-  spmv_walk<BS, TILE, V>(block_walk(nblk, xcd), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
+  spmv_walk<BS, TILE>(block_walk(nblk, xcd), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
   const double t = block_sum<BS>(epi.acc, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(BS) void k_rho_commit(const double *part, int cnt, 
   const double rho = sum_partials<BS>(part, cnt, 1, red);
   if (threadIdx.x == 0) {
     ctl->rho = rho;
-    ctl->pend_rho = rho;  // the fused iteration's K_B' reads rho from here
+    ctl->pend_rho = rho;
   }
 }
 
@@ -161,7 +161,7 @@ struct PqEpi {
 
 // K_A: q = A p, partials of p.q.  Workgroup 0 commits the pending
 // iteration count and rho left by the previous iteration's last kernel.
-template <int BS, int TILE, int V>
+template <int BS, int TILE>
 __global__ __launch_bounds__(BS) void k_spmv_pq(int nblk, const int *__restrict__ rowblk,
                                                 const int *__restrict__ ia,
                                                 const int *__restrict__ ja,
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(BS) void k_spmv_pq(int nblk, const int *__restrict_
   __shared__ double prod[TILE];
   __shared__ double red[BS / kWave];
   PqEpi epi{q, p, 0.0};
-  spmv_walk<BS, TILE, V>(block_walk(nblk, xcd), rowblk, ia, ja, a, p, 1.0, prod, red, epi);
+  spmv_walk<BS, TILE>(block_walk(nblk, xcd), rowblk, ia, ja, a, p, 1.0, prod, red, epi);
   const double t = block_sum<BS>(epi.acc, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
@@ -342,38 +342,6 @@ static ViVals vi_prepare(int nnz, const double *a, int n = 0, const int *ia = nu
   return v;
 }
 
-// Jacobi K_B: alpha = rho/(p.q); x += alpha p; r -= alpha q; partials of
-// r.w (w = d_r r) and r.r
-template <int BS>
-__global__ __launch_bounds__(BS) void k_jac_update(int n, double *__restrict__ x,
-                                                   double *__restrict__ r,
-                                                   const double *__restrict__ p,
-                                                   const double *__restrict__ q,
-                                                   const double *__restrict__ d_r,
-                                                   const PcgCtl *ctl, const double *pq_part,
-                                                   int pq_cnt, double *partials) {
-  if (ctl->done) return;
-  __shared__ double red[BS / kWave];
-  const double pq = sum_partials<BS>(pq_part, pq_cnt, 1, red);
-  const double alpha = ctl->rho / pq, nalpha = -1.0 * alpha;
-  double rz = 0.0, rr = 0.0;
-  const size_t stride = (size_t)gridDim.x * BS;
-  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] + nalpha * q[i];
-    r[i] = ri;
-    const double wi = d_r[i] * ri;
-    rz += ri * wi;
-    rr += ri * ri;
-  }
-  const double t_rz = block_sum<BS>(rz, red);
-  const double t_rr = block_sum<BS>(rr, red);
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = t_rz;
-    partials[2 * blockIdx.x + 1] = t_rr;
-  }
-}
-
 // Convergence decision of iteration `it` (cg.c:106-123): -1 keep going,
 // 0 converged, 1 maxit, 2 non-finite residual.
 __device__ __forceinline__ int stop_flag(double res, int it, const PcgCtl *ctl) {
@@ -383,50 +351,15 @@ __device__ __forceinline__ int stop_flag(double res, int it, const PcgCtl *ctl) 
   return -1;
 }
 
-// Jacobi K_C: ||r||, convergence test, beta; p = beta p + d_r r
-template <int BS>
-__global__ __launch_bounds__(BS) void k_jac_p(int n, const double *__restrict__ r,
-                                              const double *__restrict__ d_r,
-                                              double *__restrict__ p, PcgCtl *ctl,
-                                              double *hist, const double *part, int cnt) {
-  if (ctl->done) return;
-  __shared__ double red[BS / kWave];
-  const double rz = sum_partials<BS>(part, cnt, 2, red);
-  const double rr = sum_partials<BS>(part + 1, cnt, 2, red);
-  const int it = ctl->iter + 1;
-  const double res = sqrt(rr);
-  const int flag = stop_flag(res, it, ctl);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    hist[it] = res;
-    ctl->res = res;
-    ctl->last_iter = it;
-    if (flag >= 0) {
-      ctl->flag = flag;
-      ctl->done = 1;
-    } else {
-      ctl->pend_iter = it;
-      ctl->pend_rho = rz;
-      ctl->pend_valid = 1;
-    }
-  }
-  if (flag >= 0) return;
-  const double beta = rz / ctl->rho;
-  const size_t stride = (size_t)gridDim.x * BS;
-  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
-    const double wi = d_r[i] * r[i];
-    p[i] = p[i] * beta + wi;  // p *= beta; p += 1.0*w  (cg.c:80-81)
-  }
-}
-
-// ------------------------------------- Jacobi iteration, x update in K_C
+// ------------------------------------------------------ Jacobi iteration
 //
-// jacobi_variant 1: the x update x += alpha p moves from K_B to K_C, which
-// reads p anyway for p = beta p + d_r r, so p is read once less per
-// iteration.  K_B leaves alpha in ctl->alpha_x; when K_C finds that the
+// The x update x += alpha p is done in K_C, which reads p anyway for
+// p = beta p + d_r r (p is read once per iteration less than with x updated
+// beside r).  K_B leaves alpha in ctl->alpha_x; when K_C finds that the
 // iteration stops it applies only the x update, and its workgroup 0 marks
 // pend_done (committed to done by the next K_A), so that no workgroup of K_C
-// can skip its rows' x update by seeing `done` early.  Same operations on
-// the same values as the three-kernel iteration: bitwise the same results.
+// can skip its rows' x update by seeing `done` early.  Same operations on the
+// same values as cg.c's order of updates.
 
 // K_B: alpha = rho/(p.q); r -= alpha q; partials of r.(d_r r) and r.r
 template <int BS>
@@ -496,134 +429,6 @@ __global__ __launch_bounds__(BS) void k_jac_px(int n, double *__restrict__ x,
     const double pi = p[i];
     x[i] += alpha * pi;                  // x += alpha*p    (cg.c:87)
     p[i] = pi * beta + d_r[i] * r[i];    // p *= beta; p += 1.0*w  (cg.c:80-81)
-  }
-}
-
-// ------------------------------------------------- fused Jacobi iteration
-//
-// jacobi_variant 2: an iteration is two launches instead of three.
-//   K_AC(t+1): the convergence test of iteration t and beta from K_B(t)'s
-//     partials; p_{t+1} = beta p_t + d_r r computed on the fly wherever the
-//     SpMV gathers it (and written once per row to the other p buffer);
-//     q = A p_{t+1} with p.q partials; and the x update of iteration t,
-//     x += alpha_t p_t, deferred from K_B(t) so p_t is read once.  If
-//     iteration t stops, K_AC only applies that x update.
-//   K_B'(t+1): alpha = rho/(p.q); r -= alpha q; r.w and r.r partials.
-// The arithmetic of every value is the unfused path's (same operations, same
-// order), so results are bitwise identical; only the traffic changes.
-// Control fields: K_AC reads rho, iter, alpha_x (committed by the previous
-// K_B') and has workgroup 0 write pend_rho, pend_iter, the history and the
-// stop state (pend_done); K_B' reads pend_rho and has workgroup 0 commit rho,
-// iter, alpha_x -- and done, once pend_done is set.
-
-struct PnewX {  // p_{t+1}[j], p *= beta; p += 1.0*w with w = d_r .* r (cg.c:80-81)
-  const double *p_old, *r, *d_r;
-  double beta;
-  __device__ double operator()(int j) const { return p_old[j] * beta + d_r[j] * r[j]; }
-};
-
-struct FusedEpi {
-  PnewX pn;
-  double *p_new, *q, *x;
-  double alpha_x, acc;
-  __device__ double init(int) const { return 0.0; }
-  __device__ void store(int row, double s) {
-    const double pv = pn(row);
-    p_new[row] = pv;
-    q[row] = s;
-    acc += pv * s;
-    x[row] += alpha_x * pn.p_old[row];  // x += alpha*p of the previous iteration
-  }
-};
-
-template <int BS, int TILE>
-__global__ __launch_bounds__(BS) void k_jac_fused(int nblk, const int *__restrict__ rowblk,
-                                                  const int *__restrict__ ia,
-                                                  const int *__restrict__ ja,
-                                                  const double *__restrict__ a,
-                                                  const double *__restrict__ p_old,
-                                                  double *__restrict__ p_new,
-                                                  double *__restrict__ q,
-                                                  const double *__restrict__ r,
-                                                  const double *__restrict__ d_r,
-                                                  double *__restrict__ x, PcgCtl *ctl,
-                                                  double *hist, const double *upd_part,
-                                                  int upd_cnt, double *part, int n, int xcd) {
-  if (ctl->done) return;
-  __shared__ double prod[TILE];
-  __shared__ double red[BS / kWave];
-  const double rz = sum_partials<BS>(upd_part, upd_cnt, 2, red);
-  const double rr = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
-  const int it = ctl->iter + 1;
-  const double res = sqrt(rr);
-  const int flag = stop_flag(res, it, ctl);
-  const double alpha_x = ctl->alpha_x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    hist[it] = res;
-    ctl->res = res;
-    ctl->last_iter = it;
-    if (flag >= 0) {
-      ctl->flag = flag;
-      ctl->pend_done = 1;
-    } else {
-      ctl->pend_iter = it;
-      ctl->pend_rho = rz;
-    }
-  }
-  if (flag >= 0) {  // iteration `it` stops: only its x update remains
-    const size_t stride = (size_t)gridDim.x * BS;
-    for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride)
-      x[i] += alpha_x * p_old[i];
-    return;
-  }
-  const double beta = rz / ctl->rho;
-  FusedEpi epi{PnewX{p_old, r, d_r, beta}, p_new, q, x, alpha_x, 0.0};
-  const BlockWalk w = block_walk(nblk, xcd);
-  for (int s = w.begin; s < w.end; s += w.step) {
-    const int b = w.block(s);
-    if (b >= nblk) continue;
-    spmv_block_rows_fx<BS, TILE>(b, rowblk, ia, ja, a, epi.pn, prod, red, epi);
-  }
-  const double t = block_sum<BS>(epi.acc, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-// K_B': alpha = pend_rho/(p.q); r -= alpha q; partials of r.(d_r r), r.r
-template <int BS>
-__global__ __launch_bounds__(BS) void k_jac_rupd(int n, double *__restrict__ r,
-                                                 const double *__restrict__ q,
-                                                 const double *__restrict__ d_r, PcgCtl *ctl,
-                                                 const double *pq_part, int pq_cnt,
-                                                 double *partials) {
-  if (ctl->done) return;
-  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-  if (ctl->pend_done) {
-    if (first) ctl->done = 1;
-    return;
-  }
-  __shared__ double red[BS / kWave];
-  const double pq = sum_partials<BS>(pq_part, pq_cnt, 1, red);
-  const double rho = ctl->pend_rho;
-  const double alpha = rho / pq, nalpha = -1.0 * alpha;
-  double rz = 0.0, rr = 0.0;
-  const size_t stride = (size_t)gridDim.x * BS;
-  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
-    const double ri = r[i] + nalpha * q[i];
-    r[i] = ri;
-    const double wi = d_r[i] * ri;
-    rz += ri * wi;
-    rr += ri * ri;
-  }
-  const double t_rz = block_sum<BS>(rz, red);
-  const double t_rr = block_sum<BS>(rr, red);
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = t_rz;
-    partials[2 * blockIdx.x + 1] = t_rr;
-  }
-  if (first) {
-    ctl->rho = rho;
-    ctl->iter = ctl->pend_iter;
-    ctl->alpha_x = alpha;
   }
 }
 
@@ -707,28 +512,14 @@ __global__ __launch_bounds__(BS) void k_conv(PcgCtl *ctl, double *hist, const do
 
 }  // namespace
 
-// SpMV-based PCG kernels, dispatched over the SpMV body variant
+// SpMV-based PCG kernels
 static void launch_init(unsigned g, const SpmvPlan &A, const int *ia, const int *ja,
                         const double *a, const double *x, const double *b, double *r,
                         double *part) {
   const int xcd = options().spmv_walk;
   const int *rb = A.rowblk->as<int>();
-#define LAP_CASE(V)                                                                       \
-  case V:                                                                                 \
-    k_init<kBlock, kSpmvTile, V><<<g, kBlock, 0, stream()>>>(A.nblocks, rb, ia, ja, a, x, b, \
-                                                             r, part, xcd);                 \
-    break;
-  switch (options().spmv_variant) {
-    LAP_CASE(1)
-    LAP_CASE(2)
-    LAP_CASE(3)
-    LAP_CASE(4)
-    LAP_CASE(5)
-    LAP_CASE(6)
-    default:
-      LAP_CASE(0)
-  }
-#undef LAP_CASE
+  k_init<kBlock, kSpmvTile><<<g, kBlock, 0, stream()>>>(A.nblocks, rb, ia, ja, a, x, b, r, part,
+                                                     xcd);
 }
 
 static void launch_spmv_pq(unsigned g, const SpmvPlan &A, const int *ia, const int *ja,
@@ -747,22 +538,8 @@ static void launch_spmv_pq(unsigned g, const SpmvPlan &A, const int *ia, const i
                                                                 xcd);
     return;
   }
-#define LAP_CASE(V)                                                                          \
-  case V:                                                                                    \
-    k_spmv_pq<kBlock, kSpmvTile, V><<<g, kBlock, 0, stream()>>>(A.nblocks, rb, ia, ja, a, p, q, \
-                                                                ctl, part, xcd);               \
-    break;
-  switch (options().spmv_variant) {
-    LAP_CASE(1)
-    LAP_CASE(2)
-    LAP_CASE(3)
-    LAP_CASE(4)
-    LAP_CASE(5)
-    LAP_CASE(6)
-    default:
-      LAP_CASE(0)
-  }
-#undef LAP_CASE
+  k_spmv_pq<kBlock, kSpmvTile><<<g, kBlock, 0, stream()>>>(A.nblocks, rb, ia, ja, a, p, q, ctl,
+                                                        part, xcd);
 }
 
 // ------------------------------------------------------ preconditioner ops
@@ -1095,9 +872,8 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
 
   double *R0 = ws.partials + kR0, *R1 = ws.partials + kR1, *R2 = ws.partials + kR2,
          *R3 = ws.partials + kR3;
-  const bool sync_free = options().sptrsv_mode != 0 &&
-                         (o.kind == P_ICHOL || o.kind == P_GS_STD);
-  if (sync_free) tri_clear_error();
+  const bool chunked = options().sptrsv_mode != 0 && (o.kind == P_ICHOL || o.kind == P_GS_STD);
+  if (chunked) tri_clear_error();
   PcgCtl h0{};
   h0.maxit = maxit;
   LAP_HIP(hipMemcpyAsync(ctl, &h0, sizeof h0, hipMemcpyHostToDevice, stream()));
@@ -1108,44 +884,21 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   LAP_HIP(hipEventRecord(ev0, stream()));
   launch_init(gA, A, ia, ja, a, x, b, r, R0);
   const bool jac = o.kind == P_JACOBI;
-  const int jv = jac ? options().jacobi_variant : 0;
-  const ViVals vv = jv == 2 ? ViVals() : vi_prepare(nnz, a, n, ia, ja);
-  const bool fused = jv == 2, xin_c = jv == 1;
-  // fused Jacobi: p alternates between two buffers (p_t is read while
-  // p_{t+1} is written); p_1 goes to pp[1]
-  double *pp[2] = {p, fused ? scratch_vec("pcg_p2", (size_t)n) : p};
+  const ViVals vv = vi_prepare(nnz, a, n, ia, ja);
   if (jac) {
-    k_start<kBlock, true><<<gV, kBlock, 0, stream()>>>(n, R0, (int)gA, tol, r, pd->d_r, pp[1],
-                                                       ctl, hist, R1);
+    k_start<kBlock, true><<<gV, kBlock, 0, stream()>>>(n, R0, (int)gA, tol, r, pd->d_r, p, ctl,
+                                                       hist, R1);
     k_rho_commit<kBlock><<<1, kBlock, 0, stream()>>>(R1, (int)gV, ctl);
-    if (fused) {  // iteration 1: q = A p_1, then K_B'
-      launch_spmv_pq(gA, A, ia, ja, a, pp[1], q, ctl, R2);
-      k_jac_rupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA, R3);
-    }
   } else {
     k_start<kBlock, false><<<gV, kBlock, 0, stream()>>>(n, R0, (int)gA, tol, r, nullptr, nullptr,
                                                         ctl, hist, nullptr);
   }
-  int s_iter = 0;  // fused: iteration index within a graph (buffer parity)
   auto iteration = [&]() {
-    if (fused) {
-      const double *p_old = pp[(s_iter + 1) & 1];
-      double *p_new = pp[s_iter & 1];
-      k_jac_fused<kBlock, kSpmvTile><<<gA, kBlock, 0, stream()>>>(
-          A.nblocks, A.rowblk->as<int>(), ia, ja, a, p_old, p_new, q, r, pd->d_r, x, ctl, hist,
-          R3, (int)gV, R2, n, options().spmv_walk);
-      k_jac_rupd<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA, R3);
-      ++s_iter;
-    } else if (xin_c) {
+    if (jac) {
       launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2, vv);
       k_jac_update_nx<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, q, pd->d_r, ctl, R2, (int)gA,
                                                            R3);
       k_jac_px<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, r, pd->d_r, p, ctl, hist, R3, (int)gV);
-    } else if (jac) {
-      launch_spmv_pq(gA, A, ia, ja, a, p, q, ctl, R2, vv);
-      k_jac_update<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, r, p, q, pd->d_r, ctl, R2,
-                                                        (int)gA, R3);
-      k_jac_p<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, pd->d_r, p, ctl, hist, R3, (int)gV);
     } else {
       prec_launch(o, r, w, done);
       k_rw<kBlock><<<gV, kBlock, 0, stream()>>>(n, r, w, ctl, R1);
@@ -1155,11 +908,10 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
       k_conv<kBlock><<<1, kBlock, 0, stream()>>>(ctl, hist, R3, (int)gV);
     }
   };
-  // launches per iteration -> iterations per graph
-  // variants that stop through pend_done are polled there (done follows in
-  // the next launch)
-  const int *poll = (fused || xin_c) ? &ctl->pend_done : done;
-  int per_iter = fused ? 2 : jac ? 3 : 6;
+  // launches per iteration -> iterations per graph.  Jacobi stops through
+  // pend_done and is polled there (done follows in the next launch).
+  const int *poll = jac ? &ctl->pend_done : done;
+  int per_iter = jac ? 3 : 6;
   if (o.kind == P_ICHOL)
     per_iter += tri_launches(*o.ic->lplan, &o.ic->lell, false) +
                 tri_launches(*o.ic->uplan, &o.ic->uell, false);
@@ -1170,7 +922,6 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
     per_iter += 8 * o.m * (o.k + 2);
   int G = std::max(1, std::min(options().graph_iters, 4096 / per_iter));
   G = std::min(G, maxit + 2);
-  if (fused) G = std::max(2, G & ~1);  // even: every batch starts on the same p buffer
 
   int status = LAP_OK;
   int *flags = pinned_flags();
@@ -1238,7 +989,7 @@ static int run_pcg(int n, int nnz, const int *ia, const int *ja, const double *a
   if (h_hist) LAP_HIP(hipMemcpy(h_hist, hist, sizeof(double) * (size_t)(h.last_iter + 1),
                                hipMemcpyDeviceToHost));
   if (status == LAP_OK && lap_last_status() == LAP_ERR_HIP) status = LAP_ERR_HIP;
-  if (sync_free && tri_error()) {
+  if (chunked && tri_error()) {
     set_error(LAP_ERR_NUMERIC, "pcg: a triangular-solve dependency wait ran out");
     tri_clear_error();
     if (status == LAP_OK) status = LAP_ERR_NUMERIC;
@@ -1323,47 +1074,6 @@ double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const do
   const unsigned g = spmv_grid(*A);
   const ViVals vv = vi_prepare(nnz, a, n, ia, ja);  // the kernel the solve would run
   auto launch = [&]() { launch_spmv_pq(g, *A, ia, ja, a, p, q, ctl, ws.partials + kR2, vv); };
-  launch();
-  hipEvent_t e0, e1;
-  LAP_HIP(hipEventCreate(&e0));
-  LAP_HIP(hipEventCreate(&e1));
-  LAP_HIP(hipEventRecord(e0, stream()));
-  for (int i = 0; i < reps; ++i) launch();
-  LAP_HIP(hipEventRecord(e1, stream()));
-  LAP_HIP(hipEventSynchronize(e1));
-  float ms = 0.f;
-  LAP_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return (double)ms / reps;
-}
-
-double lap_pcg_fused_timed(int n, int nnz, const int *ia, const int *ja, const double *a,
-                           const double *p_old, double *p_new, double *q, const double *r,
-                           const double *d_r, double *x, int reps) {
-  if (n <= 0 || reps < 1 ||
-      !check_device("lap_pcg_fused_timed",
-                    {{ia, "ia"}, {ja, "ja"}, {a, "a"}, {p_old, "p_old"}, {p_new, "p_new"},
-                     {q, "q"}, {r, "r"}, {d_r, "d_r"}, {x, "x"}}))
-    return -1.0;
-  const SpmvPlan *A = spmv_plan(n, nnz, ia, ja);
-  if (!A) return -1.0;
-  ReduceWs &ws = reduce_ws();
-  PcgCtl h{};
-  h.rho = 1.0;
-  h.tolrel = -1.0;  // never converges: every launch does the full work
-  h.maxit = 1 << 30;
-  h.alpha_x = 0.5;
-  PcgCtl *ctl = reinterpret_cast<PcgCtl *>(scratch_vec("timed_ctl", sizeof(PcgCtl) / 8 + 1));
-  double *hist = scratch_vec("timed_hist", 4);
-  LAP_HIP(hipMemcpyAsync(ctl, &h, sizeof h, hipMemcpyHostToDevice, stream()));
-  const unsigned g = spmv_grid(*A), gV = vgrid(n);
-  launch_fill(2 * (int)gV, 0.0, ws.partials + kR3);  // r.w = r.r = 0 -> beta = 0
-  auto launch = [&]() {
-    k_jac_fused<kBlock, kSpmvTile><<<g, kBlock, 0, stream()>>>(
-        A->nblocks, A->rowblk->as<int>(), ia, ja, a, p_old, p_new, q, r, d_r, x, ctl, hist,
-        ws.partials + kR3, (int)gV, ws.partials + kR2, n, options().spmv_walk);
-  };
   launch();
   hipEvent_t e0, e1;
   LAP_HIP(hipEventCreate(&e0));

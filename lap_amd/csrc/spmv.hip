@@ -29,7 +29,7 @@ struct PlainEpi {
   __device__ void store(int row, double v) const { y[row] = v; }
 };
 
-template <int BS, int TILE, int V>
+template <int BS, int TILE>
 __global__ __launch_bounds__(BS) void k_spmv(int nblk, const int *__restrict__ rowblk,
                                              const int *__restrict__ ia,
                                              const int *__restrict__ ja,
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(BS) void k_spmv(int nblk, const int *__restrict__ r
   __shared__ double prod[TILE];
   __shared__ double red[BS / kWave];
   PlainEpi epi{y, beta};
-  spmv_walk<BS, TILE, V>(block_walk(nblk, xcd), rowblk, ia, ja, a, x, alpha, prod, red, epi);
+  spmv_walk<BS, TILE>(block_walk(nblk, xcd), rowblk, ia, ja, a, x, alpha, prod, red, epi);
 }
 
 using Key = std::tuple<const int *, const int *, int, int, int>;
@@ -103,11 +103,7 @@ const SpmvPlan *spmv_plan(int n, int nnz, const int *ia, const int *ja) {
   }
   plan->nblocks = (int)starts.size() - 1;
   plan->max_row = max_row;
-  // row starts, then the nonzero offset of each (read by the prefetching body)
-  std::vector<int> rb(starts);
-  rb.reserve(2 * starts.size());
-  for (int r : starts) rb.push_back(n > 0 ? h_ia[r] : 0);
-  plan->rowblk = dev_upload(rb);
+  plan->rowblk = dev_upload(starts);
   SpmvPlan *raw = plan.get();
   g_plans[key] = std::move(plan);
   return raw;
@@ -124,22 +120,8 @@ void spmv_launch(const SpmvPlan &p, const int *ia, const int *ja, const double *
   const unsigned g = spmv_grid(p);
   const int xcd = options().spmv_walk;
   const int *rb = p.rowblk->as<int>();
-#define LAP_SPMV_CASE(V)                                                                   \
-  case V:                                                                                  \
-    k_spmv<kBlock, kSpmvTile, V><<<g, kBlock, 0, stream()>>>(p.nblocks, rb, ia, ja, a, x, y, \
-                                                             alpha, beta, xcd, done);        \
-    break;
-  switch (options().spmv_variant) {
-    LAP_SPMV_CASE(1)
-    LAP_SPMV_CASE(2)
-    LAP_SPMV_CASE(3)
-    LAP_SPMV_CASE(4)
-    LAP_SPMV_CASE(5)
-    LAP_SPMV_CASE(6)
-    default:
-      LAP_SPMV_CASE(0)
-  }
-#undef LAP_SPMV_CASE
+  k_spmv<kBlock, kSpmvTile><<<g, kBlock, 0, stream()>>>(p.nblocks, rb, ia, ja, a, x, y, alpha,
+                                                     beta, xcd, done);
 }
 
 void release_tri_plans();

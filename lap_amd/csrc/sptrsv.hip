@@ -9,50 +9,23 @@
 // computed with exactly the reference's sequential arithmetic
 // (simple_blas.c:46-85) and the result equals the sequential solve bitwise.
 //
-// Sync-free mode (option sptrsv_mode 1): one launch per solve.  Rows are cut
-// into chunks of at most 64 rows of a single level, in level order.  A wave
-// claims 4 consecutive chunks at a time from an atomic ticket and solves
-// them in order, one lane per row.  Before the solve y is filled with a
-// sentinel (a signalling-NaN bit pattern no arithmetic produces); a lane
-// reads each dependency y[c] with 64-bit agent-scope atomic loads until it
-// is not the sentinel, and stores its result with a 64-bit agent-scope
-// atomic store, so the value is its own ready flag and no fence is needed.
-// Progress: a lane only waits for rows of earlier levels, i.e. of earlier
-// chunks, which waves that are already running have claimed (tickets are
-// taken in order), and rows of one chunk never depend on each other.  Every
-// wait is bounded: past the bound (far beyond any real wait) the lane raises
-// an error flag, every other waiting lane gives up as soon as it sees it,
-// and the host reports the failure, so the kernel always ends.  The
-// arithmetic is the level-scheduled kernel's, in the same order: results
-// are bitwise identical.
-//
-// Level-barrier mode (option sptrsv_mode 2): one persistent launch per
-// solve, a grid-wide barrier between levels.  Every workgroup takes its
-// share of the current level's rows (one lane per row), stores y with
-// agent-scope stores (written through to the coherence point, so no cache
-// write-back is needed), waits until its stores are complete, and arrives at
-// a counter; wave 0 then polls the counter (wave-uniform, bounded) until all
-// workgroups have arrived, and the next level's dependencies are read with
-// agent-scope loads.  The grid is at most one workgroup per CU, so every
-// workgroup is resident; every wait is bounded and raises the shared error
-// flag when it runs out, so the kernel always ends.  Same row arithmetic as
-// the level launches: bitwise identical.
-//
-// Chunked mode (option sptrsv_mode 3): one persistent launch per solve, one
-// workgroup per chunk of consecutive rows in solve order (all workgroups
-// resident: at most one per CU).  A chunk runs its rows level by level; a
-// level of a chunk is a "step", one lane per row, rows packed 4 or 8 wide.
-// A row's dependencies lie in earlier steps of its own chunk or in earlier
-// chunks, never later ones, so the chunks form a pipeline: before a step,
-// wave 0 waits until every earlier chunk the step depends on has published
-// (in its progress counter) the highest level the step needs from it --
+// Chunked mode (option sptrsv_mode 3, the default): one persistent launch
+// per solve, one workgroup per chunk of rows (all workgroups resident: at
+// most one per CU).  A chunk runs its rows level by level; a level of a chunk
+// is a "step", one lane per row, the rows packed as fixed-size records.
+// Every dependency of a step lies in an earlier step of its own chunk (LDS
+// ring or y) or in another chunk, at a lower level; before a step, compute
+// wave 0 makes sure every chunk the step depends on has published (in its
+// progress counter) the highest level the step needs from it --
 // point-to-point waits, listed per step by the analysis and pruned to the
 // ones not already implied by an earlier wait of the chunk.  y is stored with
 // agent-scope stores and dependencies are read with agent-scope loads (both
 // coherent across the XCDs' L2s without cache write-backs); a step's stores
 // are complete (s_waitcnt) before its level is published.  Every wait is
 // bounded and raises the shared error flag when it runs out, so the kernel
-// always ends.  Same row arithmetic as the level launches: bitwise identical.
+// always ends.  Same row arithmetic as the level launches: bitwise
+// identical.  Triangles whose rows do not fit a record keep the level
+// launches (option sptrsv_mode 0 forces them everywhere).
 #include <algorithm>
 #include <atomic>
 #include <cstdlib>
@@ -264,149 +237,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_level_ell(
   y[row] = acc / piv;
 }
 
-constexpr unsigned long long kSentinel = 0x7ff44c4150534e54ull;  // sNaN payload
-constexpr int kSfChunk = 64, kSfClaim = 4;
-constexpr unsigned kSfSpinLimit = 1u << 20;
-
 __device__ __forceinline__ double sf_load(const double *p) {
   const unsigned long long v = __hip_atomic_load(
       reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return __longlong_as_double((long long)v);
-}
-
-// y[i] = sentinel for i < n; state[0] (ticket) = 0
-__global__ __launch_bounds__(kBlock) void k_sf_fill(int n, double *__restrict__ y, int *state,
-                                                    const int *__restrict__ done) {
-  if (done && *done) return;
-  const size_t stride = (size_t)gridDim.x * kBlock;
-  unsigned long long *yy = reinterpret_cast<unsigned long long *>(y);
-  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < (size_t)n; i += stride)
-    yy[i] = kSentinel;
-  if (blockIdx.x == 0 && threadIdx.x == 0) state[0] = 0;
-}
-
-// state[0]: chunk ticket; state[1]: error flag (dependency wait ran out)
-__global__ __launch_bounds__(kBlock) void k_tri_syncfree(
-    int nchunks, const int *__restrict__ chunk_ptr, const int *__restrict__ rows,
-    const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
-    const double *__restrict__ diag, const double *__restrict__ x, double *y, int *state,
-    const int *__restrict__ done, int naps) {
-  if (done && *done) return;
-  const int lane = threadIdx.x & 63;
-  for (;;) {
-    int first = 0;
-    if (lane == 0) first = atomicAdd(&state[0], kSfClaim);
-    first = __builtin_amdgcn_readfirstlane(__shfl(first, 0));
-    if (first >= nchunks) break;
-    const int last = min(first + kSfClaim, nchunks);
-    for (int c = first; c < last; ++c) {
-      const int t = chunk_ptr[c] + lane;
-      if (t >= chunk_ptr[c + 1]) continue;
-      const int row = rows[t];
-      double acc = x[row];
-      double piv = 0.0;
-      bool in_row = false;
-      const int e1 = ia[row + 1];
-      for (int e = ia[row]; e < e1; ++e) {
-        const int col = ja[e];
-        if (col == row) {
-          piv = a[e];
-          in_row = true;
-          continue;
-        }
-        double v = sf_load(y + col);
-        unsigned spins = 0;
-        while (__double_as_longlong(v) == (long long)kSentinel) {
-          if (++spins > kSfSpinLimit ||
-              __hip_atomic_load(&state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            __hip_atomic_store(&state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v = __builtin_nan("");
-            break;
-          }
-          for (int z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(8);  // ~512 cycles each
-          v = sf_load(y + col);
-        }
-        acc -= a[e] * v;
-      }
-      if (!in_row && diag) piv = diag[row];
-      const double r = acc / piv;
-      __hip_atomic_store(reinterpret_cast<unsigned long long *>(y + row),
-                         (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
-// state[2]: barrier arrivals (reset before every solve)
-__global__ void k_tb_reset(int *state, const int *__restrict__ done) {
-  if (done && *done) return;
-  if (threadIdx.x == 0)
-    __hip_atomic_store(&state[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-constexpr unsigned kTbSpinLimit = 1u << 22;
-
-template <int BS>
-__global__ __launch_bounds__(BS) void k_tri_barrier(
-    int nlevels, const int *__restrict__ level_ptr, const int *__restrict__ rows,
-    const int *__restrict__ ia, const int *__restrict__ ja, const double *__restrict__ a,
-    const double *__restrict__ diag, const double *__restrict__ x, double *y, int *state,
-    const int *__restrict__ done) {
-  if (done && *done) return;
-  __shared__ int failed;
-  const int G = gridDim.x;
-  for (int l = 0; l < nlevels; ++l) {
-    const int l0 = level_ptr[l], l1 = level_ptr[l + 1];
-    for (int t = l0 + blockIdx.x * BS + (int)threadIdx.x; t < l1; t += G * BS) {
-      const int row = rows[t];
-      double acc = x[row];
-      double piv = 0.0;
-      bool in_row = false;
-      const int e1 = ia[row + 1];
-      for (int e = ia[row]; e < e1; ++e) {
-        const int c = ja[e];
-        if (c == row) {
-          piv = a[e];
-          in_row = true;
-          continue;
-        }
-        acc -= a[e] * sf_load(y + c);
-      }
-      if (!in_row && diag) piv = diag[row];
-      const double r = acc / piv;
-      __hip_atomic_store(reinterpret_cast<unsigned long long *>(y + row),
-                         (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (l + 1 == nlevels) break;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
-    __syncthreads();
-    if (threadIdx.x < kWave) {  // wave 0: arrive, then wait for every workgroup
-      if (threadIdx.x == 0) atomicAdd(&state[2], 1);
-      const int target = (l + 1) * G;
-      unsigned spins = 0;
-      int ok = 1;
-      for (;;) {
-        const int cur = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(&state[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        const int err = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(&state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (cur >= target) break;
-        if (err || ++spins > kTbSpinLimit) {
-          ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-      if (threadIdx.x == 0) {
-        if (!ok)
-          __hip_atomic_store(&state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        failed = !ok;
-      }
-    }
-    __syncthreads();
-    if (failed) break;
-  }
 }
 
 // ---------------------------------------------------------- chunked mode
@@ -718,7 +552,7 @@ int *ch_progress() {
   return buf && buf->p ? buf->as<int>() : nullptr;
 }
 
-int *sf_state() {  // [ticket, error, barrier arrivals, pad]
+int *sf_state() {  // [unused, error flag, unused, unused]
   static DevBufPtr buf;
   if (!buf) {
     buf = dev_alloc(4 * sizeof(int));
@@ -1055,7 +889,6 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
     p->max_level_rows = std::max(p->max_level_rows, p->level_ptr[l + 1] - p->level_ptr[l]);
   for (int i = 0; i < n; ++i) p->max_row = std::max(p->max_row, ia[i + 1] - ia[i]);
   p->order = dev_upload(order);
-  p->level_ptr_dev = dev_upload(p->level_ptr);
   std::vector<int> desc((size_t)n * 4);
   for (int t = 0; t < n; ++t) {
     desc[(size_t)t * 4] = order[(size_t)t];
@@ -1064,14 +897,6 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
     desc[(size_t)t * 4 + 3] = 0;
   }
   p->desc = dev_upload(desc);
-  // sync-free chunks: <= kSfChunk rows, never spanning two levels
-  std::vector<int> cp;
-  cp.reserve((size_t)n / kSfChunk + (size_t)nlev + 1);
-  for (int l = 0; l < nlev; ++l)
-    for (int t = p->level_ptr[l]; t < p->level_ptr[l + 1]; t += kSfChunk) cp.push_back(t);
-  cp.push_back(n);
-  p->nchunks = (int)cp.size() - 1;
-  p->chunk_ptr = dev_upload(cp);
   tri_chunks_build(*p, ia, ja, level, order);
   return p;
 }
@@ -1143,25 +968,21 @@ void tri_ell_build(const TriPlan &p, const int *ja, const double *a, TriEll &e) 
 }
 
 // Which path tri_solve_launch takes.
-enum TriPath { TRI_CHUNKS, TRI_LEVELS_ELL, TRI_BARRIER, TRI_SYNCFREE, TRI_LEVELS };
+enum TriPath { TRI_CHUNKS, TRI_LEVELS_ELL, TRI_LEVELS };
 
 static TriPath tri_path(const TriPlan &p, const TriEll *ell, bool in_place) {
   const int mode = options().sptrsv_mode;
   if (mode == 3 && ell && ell->w && ell->crec && ell->xc && p.ch.nchunks > 0 && p.ch.w == 4 &&
       sf_state() && ch_progress())
     return TRI_CHUNKS;
-  if (ell && ell->w && (mode == 0 || mode == 3)) return TRI_LEVELS_ELL;
-  // in place (x == y) needs x intact while y is filled: level launches then
-  if (mode == 2 && !in_place && p.nlevels > 0 && sf_state()) return TRI_BARRIER;
-  if (mode == 1 && !in_place && p.n > 0 && sf_state()) return TRI_SYNCFREE;
+  (void)in_place;  // the chunked kernel reads x from its own gathered copy
+  if (ell && ell->w) return TRI_LEVELS_ELL;
   return TRI_LEVELS;
 }
 
 int tri_launches(const TriPlan &p, const TriEll *ell, bool in_place) {
   switch (tri_path(p, ell, in_place)) {
     case TRI_CHUNKS: return 3;
-    case TRI_BARRIER: return 2;
-    case TRI_SYNCFREE: return 2;
     default: return p.nlevels;
   }
 }
@@ -1202,29 +1023,6 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
                                                        ell->a->as<double>() + off, diag, x, y,
                                                        done);
     }
-    return;
-  }
-  if (path == TRI_BARRIER) {
-    int *st = sf_state();
-    {
-      k_tb_reset<<<1, kWave, 0, stream()>>>(st, done);
-      const unsigned g = (unsigned)std::max(
-          1, std::min((p.max_level_rows + kBlock - 1) / kBlock, device_cus()));
-      k_tri_barrier<kBlock><<<g, kBlock, 0, stream()>>>(p.nlevels, p.level_ptr_dev->as<int>(),
-                                                        order, ia, ja, a, diag, x, y, st, done);
-      return;
-    }
-  }
-  if (path == TRI_SYNCFREE) {
-    int *st = sf_state();
-    const unsigned gf = grid_for((size_t)p.n, 4, kBlock, 2048);
-    k_sf_fill<<<gf, kBlock, 0, stream()>>>(p.n, y, st, done);
-    const int per_cu = std::max(1, std::min(16, options().sf_wg_per_cu));
-    const unsigned g =
-        (unsigned)std::max(1, std::min((p.nchunks + 3) / 4, per_cu * device_cus()));
-    k_tri_syncfree<<<g, kBlock, 0, stream()>>>(p.nchunks, p.chunk_ptr->as<int>(), order, ia, ja,
-                                               a, diag, x, y, st, done,
-                                               std::max(1, options().sf_naps));
     return;
   }
   const int4 *desc = p.desc->as<int4>();

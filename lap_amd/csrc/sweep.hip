@@ -69,7 +69,7 @@ __global__ __launch_bounds__(BS) void k_sweep_one(int nblk, const int *__restric
   __shared__ double prod[TILE];
   __shared__ double red[BS / kWave];
   SweepEpi<MODE> epi{base, d_r, out, out2, first};
-  spmv_walk<BS, TILE, 0>(block_walk(nblk, xcd), rowblk, ia, ja, a, x, -1.0, prod, red, epi);
+  spmv_walk<BS, TILE>(block_walk(nblk, xcd), rowblk, ia, ja, a, x, -1.0, prod, red, epi);
 }
 
 // Sum of a[e] * x[ja[e]] over [e0, e1) from 0.0, as the plain SpMV sums one

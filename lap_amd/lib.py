@@ -100,8 +100,6 @@ _SIGS = {
     "lap_csr_matmat": (C.c_int, [C.c_int, C.c_int, PI, PI, PD, C.c_int, PD, PD, C.c_int]),
     "lap_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
     "lap_pcg_spmv_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, C.c_int]),
-    "lap_pcg_fused_timed": (C.c_double, [C.c_int, C.c_int, PI, PI, PD, PD, PD, PD, PD, PD, PD,
-                                         C.c_int]),
     "lap_set_option": (C.c_int, [C.c_char_p, C.c_int]),
     # input path (include/lap_io.h)
     "lap_mm_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(VP), C.POINTER(MmInfo)]),

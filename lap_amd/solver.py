@@ -233,18 +233,6 @@ def spmv_timed(A: DeviceCSR, x: torch.Tensor, y: torch.Tensor, reps: int,
     return ms
 
 
-def fused_timed(A: DeviceCSR, d_r: torch.Tensor, reps: int) -> float:
-    """Average device ms per launch of the fused Jacobi iteration's first
-    kernel (p update in the gather, q = A p with p.q, deferred x update)."""
-    n = A.n
-    v = [torch.randn(n, dtype=torch.float64, device=DEV) for _ in range(5)]
-    _sync()
-    ms = L.load().lap_pcg_fused_timed(n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a), ptr(v[0]),
-                                      ptr(v[1]), ptr(v[2]), ptr(v[3]), ptr(d_r), ptr(v[4]), reps)
-    L.check()
-    return ms
-
-
 @dataclass
 class MixedResult(Result):
     refinements: int = 0

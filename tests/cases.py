@@ -56,3 +56,22 @@ def ring_chords(n, per, seed):
     lo, hi = np.minimum(src, dst), np.maximum(src, dst)
     key = np.unique(lo.astype(np.int64) * n + hi)
     return n, (key // n).astype(np.int32), (key % n).astype(np.int32)
+
+
+def nine_point(nx, ny, shift=1e-8):
+    """2D 9-point graph Laplacian: L+D rows of up to 5 entries (the 8-wide
+    packed rows of the triangular solves)."""
+    from lap_amd.problems import from_coo
+    idx = np.arange(nx * ny).reshape(ny, nx)
+    rows, cols = [], []
+    for dy, dx in ((0, 1), (1, -1), (1, 0), (1, 1)):
+        a = idx[max(0, -dy):ny - max(0, dy), max(0, -dx):nx - max(0, dx)]
+        b = idx[max(0, dy):ny - max(0, -dy) if dy <= 0 else ny, max(0, dx):nx - max(0, -dx)]
+        a, b = a.ravel(), b.ravel()
+        rows.append(np.maximum(a, b))
+        cols.append(np.minimum(a, b))
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    deg = np.bincount(r, minlength=nx * ny) + np.bincount(c, minlength=nx * ny)
+    n = nx * ny
+    return from_coo(n, np.concatenate([r, np.arange(n)]), np.concatenate([c, np.arange(n)]),
+                    np.concatenate([-np.ones(r.shape[0]), deg + shift]), symmetric=True)

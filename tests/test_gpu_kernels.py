@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import torch
 
-from cases import PROBLEMS, star_graph
+from cases import PROBLEMS, nine_point, star_graph
 from lap_amd import solver as S
 from lap_amd.problems import CSR, augment_lower, augment_upper, chung_lu, grid_laplacian, split
 
@@ -61,21 +61,16 @@ def test_spmv_long_rows(orc):
     np.testing.assert_array_equal(y[1:], want[1:])
 
 
-@pytest.mark.parametrize("rows,walk,variant", [(64, 1, 0), (512, 2, 0), (256, 0, 1), (1, 1, 2),
-                                               (256, 2, 3), (128, 0, 4), (256, 1, 5), (256, 0, 6),
-                                               (512, 2, 6), (256, 3, 0), (64, 4, 6), (1, 5, 2),
-                                               (128, 3, 4), (16, 5, 5)])
-def test_spmv_blocking_invariant(orc, rows, walk, variant):
-    """Row-block size, walk order and body variant change scheduling, never
-    the bits."""
+@pytest.mark.parametrize("rows,walk", [(64, 0), (512, 0), (1, 0), (256, 3), (16, 3), (512, 3)])
+def test_spmv_blocking_invariant(orc, rows, walk):
+    """Row-block size and walk order change scheduling, never the bits."""
     P = PROBLEMS["graph_big"]()
     x = np.random.default_rng(3).standard_normal(P.n)
-    old = [S.set_option("spmv_rows", rows), S.set_option("spmv_walk", walk),
-           S.set_option("spmv_variant", variant)]
+    old = [S.set_option("spmv_rows", rows), S.set_option("spmv_walk", walk)]
     try:
         y = host(S.spmv(S.DeviceCSR.from_csr(P), dev(x)))
     finally:
-        for name, v in zip(("spmv_rows", "spmv_walk", "spmv_variant"), old):
+        for name, v in zip(("spmv_rows", "spmv_walk"), old):
             S.set_option(name, v)
     np.testing.assert_array_equal(y, orc.matvec(P, x))
 
@@ -245,34 +240,27 @@ def test_entry_points_refuse_host_vectors():
         L.check()
 
 
-@pytest.fixture(params=[1, 2], ids=["syncfree", "levelbarrier"])
-def sync_free(request):
-    """Triangular solves in one launch for the duration of a test: mode 1
-    (sync-free, dependency waits on the values themselves) or mode 2 (one
-    grid barrier per level)."""
+@pytest.fixture(params=[0, 3], ids=["levels", "chunked"])
+def tri_mode(request):
+    """Triangular solves for the duration of a test: one launch per level
+    (mode 0) or the chunked persistent solve (mode 3, where the rows fit its
+    records: the IC0 factor's triangles)."""
     prev = S.set_option("sptrsv_mode", request.param)
-    yield
+    yield request.param
     S.set_option("sptrsv_mode", prev)
 
 
-def test_trisolve_sync_free_bitwise(orc, problem, sync_free):
-    """Same bits as the sequential solve (and so as the level-scheduled
-    launches), both conventions, lower and upper."""
-    print(f"[case] sync-free trisolve n={problem.n}", flush=True)
-    test_trisolve_bitwise(orc, problem)
-
-
 @pytest.mark.parametrize("kind", ["GS_std", "ichol"])
-def test_prec_sync_free_bitwise(orc, problem, kind, sync_free):
-    print(f"[case] sync-free prec {kind} n={problem.n}", flush=True)
+def test_prec_tri_modes_bitwise(orc, problem, kind, tri_mode):
+    print(f"[case] prec {kind} n={problem.n} sptrsv_mode={tri_mode}", flush=True)
     test_prec_bitwise(orc, problem, kind)
 
 
-def test_trisolve_sync_free_large_bitwise(orc, sync_free):
-    """A 48^3 grid (142 levels, ~1700 chunks): every dependency crosses
-    chunks claimed by other waves."""
-    A = grid_laplacian([48, 48, 48], 1e-8)
-    test_trisolve_bitwise(orc, A)
+@pytest.mark.parametrize("kind", ["GS_std", "ichol"])
+def test_prec_nine_point_bitwise(orc, kind, tri_mode):
+    """The 9-point stencil's triangles hold rows of 5 entries: packed 8 wide
+    for the level launches, and chunked (4 off-diagonal codes) in mode 3."""
+    test_prec_bitwise(orc, nine_point(90, 70), kind)
 
 
 LONG_ROWS = {"power_law": lambda: chung_lu(16384, seed=7), "star": lambda: star_graph(6000)}
@@ -302,12 +290,6 @@ def test_trisolve_long_rows_bitwise(orc, name):
             np.testing.assert_array_equal(got, want)
         else:
             np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
-
-
-def test_trisolve_long_rows_one_launch_bitwise(orc, sync_free):
-    A = LONG_ROWS["power_law"]()
-    print(f"[case] long rows one-launch n={A.n}", flush=True)
-    test_trisolve_bitwise(orc, A)
 
 
 @pytest.fixture

@@ -5,8 +5,8 @@ the matrix's distinct column offsets col - row).
 Bars: with the values stored as one-byte indices into a table of the
 matrix's distinct doubles (and, format 2, the columns as indices into the
 table of offsets), every PCG solve is bitwise the CSR solve (x, residual
-history, iteration count) -- for Jacobi in both three-launch iteration
-variants and for IC0 and GS_std, whose A products use it too; a matrix with
+history, iteration count) -- for Jacobi and for IC0 and GS_std, whose A
+products use it too; a matrix with
 255 distinct values is encoded, one with 256 is not (the solve then uses the
 CSR values, again bitwise the same); format 2 falls back to format 1 when the
 offsets do not fit (a random graph) or a row is longer than 64 entries.
@@ -45,11 +45,10 @@ def graph_with_k_values(n, k, seed):
     return from_coo(n, rows, cols, np.concatenate([-w, deg + 1e-3]), symmetric=True)
 
 
-def solve_both(Ah, kind, variant=1, fmt=1):
+def solve_both(Ah, kind, fmt=1):
     A = S.DeviceCSR.from_csr(Ah)
     P = S.make_preconditioner(A, Ah, kind)
     b = S.to_dev(rhs_normal(Ah.n, 21))
-    prev = S.set_option("jacobi_variant", variant)
     try:
         S.set_option("spmv_format", 0)
         r0 = S.pcg(A, b, P, tol=1e-10, maxit=5000)
@@ -57,7 +56,6 @@ def solve_both(Ah, kind, variant=1, fmt=1):
         r1 = S.pcg(A, b, P, tol=1e-10, maxit=5000)
     finally:
         S.set_option("spmv_format", 0)
-        S.set_option("jacobi_variant", prev)
     return r0, r1
 
 
@@ -68,11 +66,10 @@ def same(r0, r1):
 
 
 @pytest.mark.parametrize("fmt", [1, 2])
-@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("pname", ["grid3d_16", "grid2d"])
-def test_value_indexed_jacobi_pcg_bitwise(pname, variant, fmt):
-    r0, r1 = solve_both(PROBLEMS[pname](), "line_jacobi", variant, fmt)
-    print(f"[diag] format {fmt} {pname} v{variant}: {r1.iterations} it, csr {r0.gpu_ms:.2f} ms, "
+def test_value_indexed_jacobi_pcg_bitwise(pname, fmt):
+    r0, r1 = solve_both(PROBLEMS[pname](), "line_jacobi", fmt)
+    print(f"[diag] format {fmt} {pname}: {r1.iterations} it, csr {r0.gpu_ms:.2f} ms, "
           f"indexed {r1.gpu_ms:.2f} ms", flush=True)
     assert same(r0, r1)
 

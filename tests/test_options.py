@@ -12,18 +12,14 @@ pytestmark = pytest.mark.skipif(not os.path.exists(L.LIB_PATH), reason="library 
 # name: (valid values, an invalid value or None, default)
 OPTIONS = {
     "spmv_rows": ([1, 256, 512], 513, 256),
-    "spmv_walk": ([0, 5], 6, 0),
-    "spmv_variant": ([0, 6], 7, 0),
-    "jacobi_variant": ([0, 2], 3, 1),
+    "spmv_walk": ([0, 3], 1, 0),
     "fuse_sweeps": ([0, 1], 2, 1),
     "spmv_format": ([0, 1, 2], 3, 0),
     "mixed_vectors": ([0, 1], 2, 0),
-    "sptrsv_mode": ([0, 1, 2, 3], 4, 3),
+    "sptrsv_mode": ([0, 3], 1, 3),
     "sptrsv_long": ([0, 1], -1, 0),
     "spmm_wg_per_cu": ([1, 16], 17, None),
     "spmv_wg_per_cu": ([1, 16], 0, 8),
-    "sf_wg_per_cu": ([1, 16], 0, None),
-    "sf_naps": ([1, 64], 65, None),
 }
 
 
@@ -54,4 +50,8 @@ def test_option_range(name):
 
 
 def test_unknown_option():
-    assert L.load().lap_set_option(b"no_such_option", 1) == -1
+    lib = L.load()
+    assert lib.lap_set_option(b"no_such_option", 1) == -1
+    # options of measured-and-dropped variants are gone
+    for name in (b"spmv_variant", b"jacobi_variant", b"sf_wg_per_cu", b"sf_naps"):
+        assert lib.lap_set_option(name, 0) == -1
