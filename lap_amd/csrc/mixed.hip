@@ -334,19 +334,28 @@ __global__ __launch_bounds__(BS) void k_mx_update(int n, VT *__restrict__ x32,
   }
 }
 
-// refinement, step 1: x += x32, x32 = 0 (only when this iteration refines)
+// refinement, step 1: x += x32, x32 = 0 (only when this iteration refines);
+// with `xsum` given, partial sums of the new x (for the deflation shift)
 template <int BS, class VT>
 __global__ __launch_bounds__(BS) void k_mx_fold(int n, double *__restrict__ x,
                                                 VT *__restrict__ x32, const MixCtl *ctl,
-                                                const double *upd_part, int upd_cnt) {
+                                                const double *upd_part, int upd_cnt,
+                                                double *xsum) {
   if (ctl->done) return;
   __shared__ double red[BS / kWave];
   const double rr = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
   if (!need_refine(rr, ctl->iter + 1, ctl)) return;
+  double s = 0.0;
   const size_t stride = (size_t)gridDim.x * BS;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
-    x[i] += (double)x32[i];
+    const double v = x[i] + (double)x32[i];
+    x[i] = v;
     x32[i] = VT(0);
+    s += v;
+  }
+  if (xsum) {
+    const double t = block_sum<BS>(s, red);
+    if (threadIdx.x == 0) xsum[blockIdx.x] = t;
   }
 }
 
@@ -443,35 +452,147 @@ __global__ __launch_bounds__(BS) void k_mx_jac_p(int n, const VT *__restrict__ r
   }
 }
 
-// generic: partials of r.w
-template <int BS, class VT>
+// generic: partials of r.w; deflated (DEFL): [r.w, sum w, sum r] per
+// workgroup
+template <int BS, class VT, bool DEFL>
 __global__ __launch_bounds__(BS) void k_mx_rw(int n, const VT *__restrict__ r,
                                               const VT *__restrict__ w, const MixCtl *ctl,
                                               double *part) {
   if (ctl->done) return;
   __shared__ double red[BS / kWave];
-  double s = 0.0;
+  double s = 0.0, sw = 0.0, sr = 0.0;
   const size_t stride = (size_t)gridDim.x * BS;
-  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride)
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
     s += (double)r[i] * (double)w[i];
+    if (DEFL) {
+      sw += (double)w[i];
+      sr += (double)r[i];
+    }
+  }
   const double t = block_sum<BS>(s, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (!DEFL) {
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    return;
+  }
+  const double tw = block_sum<BS>(sw, red);
+  const double tr = block_sum<BS>(sr, red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = t;
+    part[3 * blockIdx.x + 1] = tw;
+    part[3 * blockIdx.x + 2] = tr;
+  }
 }
 
-// generic: rho, p = w (first) or beta p + w
-template <int BS, class VT>
+// generic: rho, p = w (first) or beta p + w.  Deflated (DEFL): w is taken
+// with its mean removed (w - mu, mu = sum w / n), so p, q = A p and r stay
+// orthogonal to the constant vector, and rho = r.(w - mu) = r.w - mu sum r.
+template <int BS, class VT, bool DEFL>
 __global__ __launch_bounds__(BS) void k_mx_pupd(int n, const VT *__restrict__ w,
                                                 VT *__restrict__ p, MixCtl *ctl,
                                                 const double *part, int cnt) {
   if (ctl->done) return;
   __shared__ double red[BS / kWave];
-  const double rho = sum_partials<BS>(part, cnt, 1, red);
+  double rho, mu = 0.0;
+  if (DEFL) {
+    const double rw = sum_partials<BS>(part, cnt, 3, red);
+    const double sw = sum_partials<BS>(part + 1, cnt, 3, red);
+    const double sr = sum_partials<BS>(part + 2, cnt, 3, red);
+    mu = sw / (double)n;
+    rho = rw - mu * sr;
+  } else {
+    rho = sum_partials<BS>(part, cnt, 1, red);
+  }
   const bool first = ctl->iter == 0;
   const double beta = first ? 0.0 : rho / ctl->rho_prev;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->rho = rho;
   const size_t stride = (size_t)gridDim.x * BS;
-  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride)
-    p[i] = first ? w[i] : (VT)((double)p[i] * beta + (double)w[i]);
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+    const double wi = DEFL ? (double)w[i] - mu : (double)w[i];
+    p[i] = first ? (VT)wi : (VT)((double)p[i] * beta + wi);
+  }
+}
+
+// Deflation of the constant vector, for matrices whose rows all sum to the
+// same s > 0 (a graph Laplacian plus s I): A 1 = s 1, so the solution's
+// constant component is mean(b) / s and the rest lies in 1-perp, where the
+// inner iteration is kept.  x's mean is set to `target` = mean(b) / s
+// before the first residual and at every refinement.
+// Partial sums of the caller's initial x.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_xsum(int n, const double *__restrict__ x,
+                                                double *part) {
+  __shared__ double red[BS / kWave];
+  double s = 0.0;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) s += x[i];
+  const double t = block_sum<BS>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// x += target - mean(x): before the first residual, and after the fold on
+// refinement iterations (`upd_part` given).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_xshift(int n, double *__restrict__ x, double target,
+                                                  const MixCtl *ctl, const double *upd_part,
+                                                  int upd_cnt, const double *part, int cnt) {
+  if (ctl->done) return;
+  __shared__ double red[BS / kWave];
+  if (upd_part) {
+    const double rr = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
+    if (!need_refine(rr, ctl->iter + 1, ctl)) return;
+  }
+  const double d = target - sum_partials<BS>(part, cnt, 1, red) / (double)n;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) x[i] += d;
+}
+
+// Maximum over a block of BS threads; the result is valid in thread 0.
+template <int BS>
+__device__ __forceinline__ double block_max(double v, double *lds) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o));
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double r = -INFINITY;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < BS / kWave; ++i) r = fmax(r, lds[i]);
+  }
+  __syncthreads();
+  return r;
+}
+
+// Row statistics of A for the deflation test: per workgroup the smallest
+// and largest row sum (CSR order), the largest diagonal and the sum of b.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mx_rowstats(int n, const int *__restrict__ ia,
+                                                    const int *__restrict__ ja,
+                                                    const double *__restrict__ a,
+                                                    const double *__restrict__ b,
+                                                    double *part) {
+  __shared__ double red[BS / kWave];
+  double lo = INFINITY, hi = -INFINITY, dmax = 0.0, sb = 0.0;
+  const size_t stride = (size_t)gridDim.x * BS;
+  for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+    double rs = 0.0;
+    for (int e = ia[i]; e < ia[i + 1]; ++e) {
+      rs += a[e];
+      if (ja[e] == (int)i) dmax = fmax(dmax, a[e]);
+    }
+    lo = fmin(lo, rs);
+    hi = fmax(hi, rs);
+    sb += b[i];
+  }
+  const double t_lo = -block_max<BS>(-lo, red);
+  const double t_hi = block_max<BS>(hi, red);
+  const double t_d = block_max<BS>(dmax, red);
+  const double t_b = block_sum<BS>(sb, red);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = t_lo;
+    part[4 * blockIdx.x + 1] = t_hi;
+    part[4 * blockIdx.x + 2] = t_d;
+    part[4 * blockIdx.x + 3] = t_b;
+  }
 }
 
 // generic, one workgroup: convergence bookkeeping
@@ -778,6 +899,34 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
     return LAP_ERR_ARG;
   }
   const int walk = options().spmv_walk;
+  // Deflation of the constant vector (generic path): when every row of A sums
+  // to the same s > 0 and s is tiny against the diagonal (a graph Laplacian
+  // plus a small shift), A 1 = s 1 is a near-null mode that the FP32
+  // residual's rounding feeds and an IC0 preconditioner amplifies by ~1/s;
+  // the inner iteration is then kept orthogonal to 1 and x's constant
+  // component set directly to mean(b) / s.
+  bool defl = false;
+  double defl_target = 0.0;
+  const unsigned gVd = std::min<unsigned>(gV, kMaxPartialBlocks / 3);
+  if (!jac && n > 0) {
+    constexpr unsigned gs = 256;
+    double *st = scratch_vec("mx_rowstats", 4 * gs);
+    k_mx_rowstats<kBlock><<<gs, kBlock, 0, stream()>>>(n, ia, ja, a, b, st);
+    std::vector<double> h(4 * gs);
+    if (LAP_HIP(hipMemcpy(h.data(), st, sizeof(double) * h.size(), hipMemcpyDeviceToHost))) {
+      double lo = INFINITY, hi = -INFINITY, dmax = 0.0, sb = 0.0;
+      for (unsigned k = 0; k < gs; ++k) {
+        lo = std::min(lo, h[4 * k]);
+        hi = std::max(hi, h[4 * k + 1]);
+        dmax = std::max(dmax, h[4 * k + 2]);
+        sb += h[4 * k + 3];
+      }
+      if (lo > 0.0 && hi - lo <= 1e-6 * hi && hi <= 1e-6 * dmax) {
+        defl = true;
+        defl_target = (sb / n) / (0.5 * (lo + hi));
+      }
+    }
+  }
 
   MixCtl h0{};
   h0.maxit = maxit;
@@ -787,6 +936,11 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
   LAP_HIP(hipEventCreate(&ev1));
   LAP_HIP(hipEventRecord(ev0, stream()));
   k_zero_v<VT><<<gV, kBlock, 0, stream()>>>(n, x32);
+  if (defl) {  // x's constant component is known: set it
+    k_mx_xsum<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, Q3);
+    k_mx_xshift<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, defl_target, ctl, nullptr, 0, Q3,
+                                                     (int)gV);
+  }
   k_mx_init<kBlock, kSpmvTile, VT><<<gA, kBlock, 0, stream()>>>(
       A64->nblocks, A64->rowblk->as<int>(), ia, ja, a, x, b, r32, d_r32, jac ? p32 : nullptr,
       Q0, walk);
@@ -818,7 +972,7 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
       spmv32();
       k_mx_update<kBlock, true, VT><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, d_r32,
                                                                  ctl, Q1, (int)gO, Q2);
-      k_mx_fold<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
+      k_mx_fold<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV, nullptr);
       k_mx_resid<kBlock, kSpmvTile, VT><<<gA, kBlock, 0, stream()>>>(
           A64->nblocks, rb_a, ia, ja, a, x, b, r32, d_r32, ctl, Q2, (int)gV, Q3, walk);
       k_mx_jac_p<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, r32, d_r32, p32, ctl, hist, Q2,
@@ -829,18 +983,29 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
       } else {
         k_copy_v<VT><<<gV, kBlock, 0, stream()>>>(n, r32, w32, done);
       }
-      k_mx_rw<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, r32, w32, ctl, Q0);
-      k_mx_pupd<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, w32, p32, ctl, Q0, (int)gV);
+      if (defl) {
+        k_mx_rw<kBlock, VT, true><<<gVd, kBlock, 0, stream()>>>(n, r32, w32, ctl, Q0);
+        k_mx_pupd<kBlock, VT, true><<<gVd, kBlock, 0, stream()>>>(n, w32, p32, ctl, Q0,
+                                                                  (int)gVd);
+      } else {
+        k_mx_rw<kBlock, VT, false><<<gV, kBlock, 0, stream()>>>(n, r32, w32, ctl, Q0);
+        k_mx_pupd<kBlock, VT, false><<<gV, kBlock, 0, stream()>>>(n, w32, p32, ctl, Q0,
+                                                                  (int)gV);
+      }
       spmv32();
       k_mx_update<kBlock, false, VT><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32,
                                                                   nullptr, ctl, Q1, (int)gO, Q2);
-      k_mx_fold<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV);
+      k_mx_fold<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV,
+                                                         defl ? Q3 : nullptr);
+      if (defl)
+        k_mx_xshift<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, defl_target, ctl, Q2, (int)gV, Q3,
+                                                         (int)gV);
       k_mx_resid<kBlock, kSpmvTile, VT><<<gA, kBlock, 0, stream()>>>(
           A64->nblocks, rb_a, ia, ja, a, x, b, r32, nullptr, ctl, Q2, (int)gV, Q3, walk);
       k_mx_conv<kBlock><<<1, kBlock, 0, stream()>>>(ctl, hist, Q2, (int)gV, Q3, (int)gA);
     }
   };
-  int per_iter = jac ? 5 : 8;
+  int per_iter = jac ? 5 : defl ? 9 : 8;
   if (ic) per_iter += icp->lplan->nlevels + icp->uplan->nlevels;
   int G = std::max(1, std::min(options().graph_iters, 4096 / per_iter));
   G = std::min(G, maxit + 2);

@@ -13,10 +13,13 @@ off-diagonals fails on: a hub row's sum moves by more than the 1e-6 shift
 and the FP32 operator turns indefinite.  With the rounding error of each
 row folded into its FP64 diagonal the inner operator keeps A's row sums.
 
-IC0 is tested on the graphs with the larger shifts only: on a Laplacian shifted by
-1e-8 the FP32 rounding of the residual puts ~6e-8 relative noise into the
-near-null constant mode, which an IC0 preconditioner (unlike Jacobi)
-amplifies by ~1e8, and the FP32 inner process stagnates."""
+On a Laplacian shifted by only 1e-8 the FP32 rounding of the residual puts
+~6e-8 relative noise into the near-null constant mode, which an IC0
+preconditioner (unlike Jacobi) amplifies by ~1e8: the solver detects matrices
+whose rows all sum to the same tiny s, keeps the inner iteration orthogonal
+to the constant vector and sets x's constant component to mean(b)/s directly
+(test_mixed_ichol_on_1e8_shifted_laplacian, also with a right-hand side whose
+mean is not zero)."""
 import numpy as np
 import pytest
 
@@ -171,3 +174,36 @@ def test_mixed_sees_values_changed_in_place(orc, kind, vectors):
         assert err <= 1e3 * 1e-10
     finally:
         S.set_option("mixed_vectors", prev)
+
+
+@pytest.mark.parametrize("dims", [[40, 40, 40], [150, 120]])
+@pytest.mark.parametrize("mean", [0.0, 1e-5])
+def test_mixed_ichol_on_1e8_shifted_laplacian(orc, dims, mean):
+    """IC0 mixed precision on a 1e-8-shifted grid Laplacian converges to the
+    FP64 tolerance (it stagnated before the constant mode was deflated); with
+    mean(b) != 0 the solution's constant component mean(b)/1e-8 is set
+    exactly and the rest agrees with the FP64 solve.  (A larger mean puts the
+    true FP64 residual of any x out of reach: with x ~ 3e7 for mean 0.3 the
+    rounding of A x alone is 1e-8 of ||b||.)"""
+    print(f"[case] mixed ichol deflated dims={dims} mean={mean}", flush=True)
+    Ah = grid_laplacian(dims, 1e-8)
+    b = rhs_normal(Ah.n, 41) + mean
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, "ichol")
+    tol = 1e-10
+    m = S.pcg_mixed(A, S.to_dev(b), P, tol=tol, maxit=5000)
+    d = S.pcg(A, S.to_dev(b), P, tol=tol, maxit=5000)
+    x, xd = m.x.cpu().numpy(), d.x.cpu().numpy()
+    true = np.linalg.norm(b - orc.matvec(Ah, x))
+    err = np.linalg.norm(centred(x) - centred(xd)) / np.linalg.norm(centred(xd))
+    cm = abs(x.mean() - xd.mean()) / max(abs(xd.mean()), 1e-300)
+    print(f"[diag] mixed ichol {dims} mean {mean}: it {m.iterations} (fp64 {d.iterations}), "
+          f"refinements {m.refinements}, flag {m.flag}, true rel res "
+          f"{true / np.linalg.norm(b):.2e}, centred err vs FP64 {err:.2e}, mean rel err {cm:.1e}",
+          flush=True)
+    assert m.flag == 0 and d.flag == 0
+    assert true < tol * np.linalg.norm(b) * 1.01
+    assert m.iterations <= 1.5 * d.iterations + 5
+    assert err <= 1e3 * tol
+    if mean:
+        assert cm <= 1e-6
