@@ -364,8 +364,18 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
 int tri_launches(const TriPlan &p, const TriEll *ell, bool in_place);
 // The same level-scheduled solve over FP32 values and vectors with FP64 row
 // sums: y = (float)((x - sum a*y) / pivot), pivot in the row.
+// The chunked schedule's records over FP32 values (for tri_solve32_launch)
+// and a buffer for the right-hand side in schedule order.
+struct TriChunks32 {
+  DevBufPtr rec, xc;
+  bool ok = false;
+};
+void tri_ch_pack32(const TriPlan &p, const int *ja, const float *a, TriChunks32 &c);
+// With `c` packed for these values and sptrsv_mode 3: one chunked launch
+// (bitwise the level launches); otherwise the level launches.
 void tri_solve32_launch(const TriPlan &p, const int *ja, const float *a, const float *x, float *y,
-                        const int *done);
+                        const int *done, const TriChunks32 *c = nullptr);
+int tri_launches32(const TriPlan &p, const TriChunks32 *c);
 // Sync-free mode: nonzero once a dependency wait ran out (the result is then
 // invalid); reset by tri_clear_error.
 int tri_error();

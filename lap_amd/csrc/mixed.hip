@@ -700,6 +700,7 @@ struct MixPlan {
   DevBufPtr a64_o, vi_o, dict64, dict_o, code_o, ddict_o;
   int nd = 0, ndl = 0;
   DevBufPtr dr32, l32, u32;  // this solve's FP32 1/diag and IC0 factor
+  TriChunks32 lch, uch;      // the factor's chunked-solve records
   const SpmvPlan *off = nullptr;
 };
 using MixKey = std::tuple<const int *, const int *, int, int>;
@@ -815,13 +816,13 @@ const double *inner_d_r<double>(MixPlan *, const double *d_r, int) {
 // vectors: the FP64 factor itself -- the level-scheduled solves are bound by
 // latency, not bytes, so FP32 factor values would save no time and their
 // rounding costs iterations on badly conditioned systems.
-inline void ic_inner(const IcPlan &P, const float *l32, const float *u32, const float *x, float *y,
-                     float *aux, const int *done) {
-  tri_solve32_launch(*P.lplan, P.lja->as<int>(), l32, x, aux, done);
-  tri_solve32_launch(*P.uplan, P.uja->as<int>(), u32, aux, y, done);
+inline void ic_inner(const IcPlan &P, const MixPlan &M, const float *l32, const float *u32,
+                     const float *x, float *y, float *aux, const int *done) {
+  tri_solve32_launch(*P.lplan, P.lja->as<int>(), l32, x, aux, done, &M.lch);
+  tri_solve32_launch(*P.uplan, P.uja->as<int>(), u32, aux, y, done, &M.uch);
 }
-inline void ic_inner(const IcPlan &P, const float *, const float *, const double *x, double *y,
-                     double *aux, const int *done) {
+inline void ic_inner(const IcPlan &P, const MixPlan &, const float *, const float *,
+                     const double *x, double *y, double *aux, const int *done) {
   ic_apply_launch(P, x, y, aux, done);
 }
 
@@ -871,6 +872,8 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
     if (sizeof(VT) == sizeof(float)) {  // this solve's FP32 copy of the factor
       l32 = to_f32(M->l32, icp->lval->as<double>(), icp->nnzL);
       u32 = to_f32(M->u32, icp->uval->as<double>(), icp->nnzL);
+      tri_ch_pack32(*icp->lplan, icp->lja->as<int>(), l32, M->lch);
+      tri_ch_pack32(*icp->uplan, icp->uja->as<int>(), u32, M->uch);
     }
   }
 
@@ -979,7 +982,7 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
                                                           (int)gV, Q3, (int)gA);
     } else {
       if (ic) {
-        ic_inner(*icp, l32, u32, r32, w32, aux32, done);
+        ic_inner(*icp, *M, l32, u32, r32, w32, aux32, done);
       } else {
         k_copy_v<VT><<<gV, kBlock, 0, stream()>>>(n, r32, w32, done);
       }
@@ -1006,7 +1009,11 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
     }
   };
   int per_iter = jac ? 5 : defl ? 9 : 8;
-  if (ic) per_iter += icp->lplan->nlevels + icp->uplan->nlevels;
+  if (ic)
+    per_iter += sizeof(VT) == sizeof(double)
+                    ? tri_launches(*icp->lplan, &icp->lell, false) +
+                          tri_launches(*icp->uplan, &icp->uell, false)
+                    : tri_launches32(*icp->lplan, &M->lch) + tri_launches32(*icp->uplan, &M->uch);
   int G = std::max(1, std::min(options().graph_iters, 4096 / per_iter));
   G = std::min(G, maxit + 2);
 

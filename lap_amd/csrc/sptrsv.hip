@@ -242,6 +242,20 @@ __device__ __forceinline__ double sf_load(const double *p) {
       reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return __longlong_as_double((long long)v);
 }
+__device__ __forceinline__ float sf_load(const float *p) {
+  const unsigned v = __hip_atomic_load(reinterpret_cast<const unsigned *>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return __uint_as_float(v);
+}
+__device__ __forceinline__ void sf_store(double *p, double v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p),
+                     (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void sf_store(float *p, float v) {
+  __hip_atomic_store(reinterpret_cast<unsigned *>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // ---------------------------------------------------------- chunked mode
 constexpr int kChMinRows = 2048;  // rows per chunk at least
@@ -281,15 +295,27 @@ typedef __attribute__((address_space(1))) void global_void_t;
 
 // The chunked schedule's records, one per row in schedule order (bytes):
 // [0, 4W) the off-diagonal dependency codes in CSR order (-1 padding, >= 0 a
-// row of y, <= -2 the LDS ring slot -(code + 2)); [4W, 4W + 4) the row;
-// [4W + 16, 12W + 16) the off-diagonal values; [12W + 16, 12W + 24) the pivot.
+// row of y, <= -2 the LDS ring slot -(code + 2)); [4W, 4W + 4) the row; then
+// the off-diagonal values and the pivot: FP64 values at [4W + 16, 12W + 16)
+// and [12W + 16, 12W + 24), FP32 values at [4W + 8, 8W + 8) and
+// [8W + 8, 8W + 12).  The size is a multiple of 16 bytes.
+template <int W, class AT>
+struct ChRec;
 template <int W>
-struct ChRec {
+struct ChRec<W, double> {
   static constexpr int bytes = 12 * W + 32;
   static constexpr int row = W;                  // int index
-  static constexpr int val = (4 * W + 16) / 8;   // double index
-  static constexpr int piv = (12 * W + 16) / 8;  // double index
+  static constexpr int val = (4 * W + 16) / 8;   // value index
+  static constexpr int piv = (12 * W + 16) / 8;  // value index
   static constexpr int dma = bytes / 4;          // 16-byte loads of 64 lanes per kBlock records
+};
+template <int W>
+struct ChRec<W, float> {
+  static constexpr int bytes = (8 * W + 12 + 15) / 16 * 16;
+  static constexpr int row = W;          // int index
+  static constexpr int val = W + 2;      // value index
+  static constexpr int piv = 2 * W + 2;  // value index
+  static constexpr int dma = bytes / 4;
 };
 
 // One workgroup per chunk: kBlock compute lanes (one row each per step) and
@@ -313,22 +339,24 @@ struct ChRec {
 //    lanes waiting, by count, for the previous step's store only and a
 //    barrier, after which the previous step's level is published: one step
 //    behind, so that no lane waits on a store it has just issued.
-template <int W, int D, int NS>
+// AT: the values' type, VT: the vectors' (FP32 values and vectors are
+// summed in FP64 and the result rounded once, as the FP32 level launches do).
+template <class AT, class VT, int W, int D, int NS>
 __global__ __launch_bounds__(kBlock + kWave) void k_tri_chunks(
     const int *__restrict__ cstep, const int *__restrict__ sptr, const int *__restrict__ slev,
     const int *__restrict__ wptr, const int *__restrict__ wchk, const int *__restrict__ wlev,
-    const char *__restrict__ rec, const double *xc, double *y,
+    const char *__restrict__ rec, const VT *xc, VT *y,
     int *progress, int *state, const int *__restrict__ done) {
   if (done && *done) return;
-  using R = ChRec<W>;
-  constexpr int XB = kBlock * 8;  // the step's right-hand sides
+  using R = ChRec<W, AT>;
+  constexpr int XB = kBlock * (int)sizeof(VT);  // the step's right-hand sides
   constexpr int SLOT = kBlock * R::bytes + XB;
   constexpr int PER = R::dma + XB / (kWave * 4);  // loads per step: records, right-hand sides
   static_assert((D - 1) * PER <= 63, "vmcnt range");
   static_assert(D < NS, "a slot in use is never refilled");
   __shared__ __attribute__((aligned(16))) char slots[NS * SLOT];
   __shared__ int seen[kChMax];  // highest level seen published, per chunk
-  __shared__ double ring[kChRing * kBlock];
+  __shared__ VT ring[kChRing * kBlock];
   __shared__ int tsptr[kChSteps + 1], tslev[kChSteps], twptr[kChSteps + 1];
   __shared__ int twchk[kChWaits], twlev[kChWaits];
   __shared__ int failed, must_wait;
@@ -442,39 +470,37 @@ __global__ __launch_bounds__(kBlock + kWave) void k_tri_chunks(
       }
       const bool has_row = tsptr[rel] + p < tsptr[rel + 1];
       int row = 0;
-      double r = 0.0;
+      VT r = VT(0);
       if (has_row) {
         const char *slot = slots + (rel % NS) * SLOT;
         const int *ri = reinterpret_cast<const int *>(slot + p * R::bytes);
-        const double *rd = reinterpret_cast<const double *>(slot + p * R::bytes);
+        const AT *rd = reinterpret_cast<const AT *>(slot + p * R::bytes);
         int cc[W];
-        double yv[W];
+        VT yv[W];
 #pragma unroll
         for (int j = 0; j < W; ++j) cc[j] = ri[j];
 #pragma unroll
         for (int j = 0; j < W; ++j)  // the dependencies on y first: one wait for all
-          yv[j] = cc[j] >= 0 ? sf_load(y + cc[j]) : 0.0;
+          yv[j] = cc[j] >= 0 ? sf_load(y + cc[j]) : VT(0);
         row = ri[R::row];
-        const double xr = reinterpret_cast<const double *>(slot + kBlock * R::bytes)[p];
-        const double piv = rd[R::piv];
+        const VT xr = reinterpret_cast<const VT *>(slot + kBlock * R::bytes)[p];
+        const double piv = (double)rd[R::piv];
 #pragma unroll
         for (int j = 0; j < W; ++j)
           if (cc[j] < -1) yv[j] = ring[-(cc[j] + 2)];
-        double acc = xr;
+        double acc = (double)xr;
 #pragma unroll
         for (int j = 0; j < W; ++j) {
           if (cc[j] == -1) break;
-          acc -= rd[R::val + j] * yv[j];
+          acc -= (double)rd[R::val + j] * (double)yv[j];
         }
-        r = acc / piv;
+        r = (VT)(acc / piv);
         ring[(rel % kChRing) * kBlock + p] = r;
       }
       if (poll_pc >= 0) atomicMax(&seen[poll_pc], poll_v);
       if (has_row) {
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __hip_atomic_store(reinterpret_cast<unsigned long long *>(y + row),
-                           (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        sf_store(y + row, r);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
       }
       // The loads of this step were consumed above, so only stores are in
@@ -505,9 +531,10 @@ __global__ __launch_bounds__(kBlock + kWave) void k_tri_chunks(
 // (workgroups are dealt to the 8 XCDs round-robin) sweeps one contiguous
 // eighth of the positions: its workgroups then read the same lines out of
 // their own L2.  The grid is a multiple of 8.
+template <class VT>
 __global__ __launch_bounds__(kBlock) void k_ch_gather(int n, const int *__restrict__ crow,
-                                                      const double *__restrict__ x,
-                                                      double *__restrict__ xc,
+                                                      const VT *__restrict__ x,
+                                                      VT *__restrict__ xc,
                                                       const int *__restrict__ done) {
   if (done && *done) return;
   const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
@@ -518,30 +545,30 @@ __global__ __launch_bounds__(kBlock) void k_ch_gather(int n, const int *__restri
 }
 
 // The records of the chunked schedule for one set of values (k_tri_chunks).
-template <int W>
+template <int W, class AT>
 __global__ __launch_bounds__(kBlock) void k_ch_pack(int n, const int *__restrict__ cmap,
                                                     const int4 *__restrict__ desc,
                                                     const int *__restrict__ ja,
-                                                    const double *__restrict__ a,
+                                                    const AT *__restrict__ a,
                                                     const int *__restrict__ code,
                                                     char *__restrict__ rec) {
-  using R = ChRec<W>;
+  using R = ChRec<W, AT>;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
     int *ri = reinterpret_cast<int *>(rec + (size_t)t * R::bytes);
-    double *rd = reinterpret_cast<double *>(rec + (size_t)t * R::bytes);
+    AT *rd = reinterpret_cast<AT *>(rec + (size_t)t * R::bytes);
     const int4 d = desc[cmap[t]];
 #pragma unroll
     for (int j = 0; j < W; ++j) ri[j] = code[(size_t)t * W + j];
     ri[R::row] = d.x;
     int k = 0;
-    double piv = 0.0;
+    AT piv = AT(0);
     for (int e = d.y; e < d.z; ++e) {
       if (ja[e] == d.x)
         piv = a[e];
       else if (k < W)
         rd[R::val + k++] = a[e];
     }
-    for (; k < W; ++k) rd[R::val + k] = 0.0;
+    for (; k < W; ++k) rd[R::val + k] = AT(0);
     rd[R::piv] = piv;
   }
 }
@@ -951,20 +978,30 @@ void tri_ell_build(const TriPlan &p, const int *ja, const double *a, TriEll &e) 
     k_ell_pack<8><<<g, kBlock, 0, stream()>>>(p.n, p.desc->as<int4>(), ja, a, e.ja->as<int>(),
                                               e.a->as<double>());
   e.w = w;
-  if (p.ch.nchunks > 0) {  // the records of the chunked schedule for these values
-    const int cw = p.ch.w;
-    const size_t rb = (size_t)(12 * cw + 32);
+  if (p.ch.nchunks > 0 && p.ch.w == 4) {  // the chunked schedule's records for these values
     // kBlock records of padding: the prefetch copies whole slots
-    e.crec = dev_alloc(rb * ((size_t)p.n + kBlock));
+    e.crec = dev_alloc((size_t)ChRec<4, double>::bytes * ((size_t)p.n + kBlock));
     e.xc = dev_alloc(sizeof(double) * ((size_t)p.n + kBlock));
     if (!e.crec || !e.crec->p || !e.xc || !e.xc->p) return;
-    if (cw == 4)
-      k_ch_pack<4><<<g, kBlock, 0, stream()>>>(p.n, p.ch.cmap->as<int>(), p.desc->as<int4>(), ja,
-                                               a, p.ch.code->as<int>(), e.crec->as<char>());
-    else
-      k_ch_pack<8><<<g, kBlock, 0, stream()>>>(p.n, p.ch.cmap->as<int>(), p.desc->as<int4>(), ja,
-                                               a, p.ch.code->as<int>(), e.crec->as<char>());
+    k_ch_pack<4, double><<<g, kBlock, 0, stream()>>>(p.n, p.ch.cmap->as<int>(),
+                                                     p.desc->as<int4>(), ja, a,
+                                                     p.ch.code->as<int>(), e.crec->as<char>());
   }
+}
+
+void tri_ch_pack32(const TriPlan &p, const int *ja, const float *a, TriChunks32 &c) {
+  c.ok = false;
+  if (p.n <= 0 || p.ch.nchunks <= 0 || p.ch.w != 4) return;
+  const size_t bytes = (size_t)ChRec<4, float>::bytes * ((size_t)p.n + kBlock);
+  if (!c.rec || c.rec->bytes < bytes) c.rec = dev_alloc(bytes);
+  const size_t xb = sizeof(float) * ((size_t)p.n + kBlock);
+  if (!c.xc || c.xc->bytes < xb) c.xc = dev_alloc(xb);
+  if (!c.rec || !c.rec->p || !c.xc || !c.xc->p) return;
+  const unsigned g = grid_for((size_t)p.n, 1, kBlock, 65536);
+  k_ch_pack<4, float><<<g, kBlock, 0, stream()>>>(p.n, p.ch.cmap->as<int>(), p.desc->as<int4>(),
+                                                  ja, a, p.ch.code->as<int>(),
+                                                  c.rec->as<char>());
+  c.ok = true;
 }
 
 // Which path tri_solve_launch takes.
@@ -998,13 +1035,14 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
       const TriPlan::Chunks &ch = p.ch;
       k_ch_reset<<<(ch.nchunks + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(ch.nchunks, prog,
                                                                              done);
-      k_ch_gather<<<2048, kBlock, 0, stream()>>>(
+      k_ch_gather<double><<<2048, kBlock, 0, stream()>>>(
           p.n, ch.crow->as<int>(), x, ell->xc->as<double>(), done);
 #define LAP_CH_ARGS                                                                            \
   ch.cstep->as<int>(), ch.sptr->as<int>(), ch.slev->as<int>(), ch.wptr->as<int>(),             \
       ch.wchk->as<int>(), ch.wlev->as<int>(), ell->crec->as<char>(),                           \
       ell->xc->as<double>(), y, prog, st, done
-      k_tri_chunks<4, 3, 4><<<ch.nchunks, kBlock + kWave, 0, stream()>>>(LAP_CH_ARGS);
+      k_tri_chunks<double, double, 4, 3, 4><<<ch.nchunks, kBlock + kWave, 0, stream()>>>(
+          LAP_CH_ARGS);
 #undef LAP_CH_ARGS
       return;
     }
@@ -1054,8 +1092,30 @@ static void tri_mixed_launch(const TriPlan &p, const int *ja, const float *a, co
   }
 }
 
+static bool tri_chunks32_ready(const TriPlan &p, const TriChunks32 *c) {
+  return c && c->ok && options().sptrsv_mode == 3 && p.ch.nchunks > 0 && p.ch.w == 4 &&
+         sf_state() && ch_progress();
+}
+
+int tri_launches32(const TriPlan &p, const TriChunks32 *c) {
+  return tri_chunks32_ready(p, c) ? 3 : p.nlevels;
+}
+
 void tri_solve32_launch(const TriPlan &p, const int *ja, const float *a, const float *x, float *y,
-                        const int *done) {
+                        const int *done, const TriChunks32 *c) {
+  if (tri_chunks32_ready(p, c)) {
+    const TriPlan::Chunks &ch = p.ch;
+    int *st = sf_state(), *prog = ch_progress();
+    k_ch_reset<<<(ch.nchunks + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(ch.nchunks, prog,
+                                                                           done);
+    k_ch_gather<float><<<2048, kBlock, 0, stream()>>>(p.n, ch.crow->as<int>(), x,
+                                                      c->xc->as<float>(), done);
+    k_tri_chunks<float, float, 4, 3, 4><<<ch.nchunks, kBlock + kWave, 0, stream()>>>(
+        ch.cstep->as<int>(), ch.sptr->as<int>(), ch.slev->as<int>(), ch.wptr->as<int>(),
+        ch.wchk->as<int>(), ch.wlev->as<int>(), c->rec->as<char>(), c->xc->as<float>(), y, prog,
+        st, done);
+    return;
+  }
   tri_mixed_launch<float>(p, ja, a, x, y, done);
 }
 

@@ -207,3 +207,28 @@ def test_mixed_ichol_on_1e8_shifted_laplacian(orc, dims, mean):
     assert err <= 1e3 * tol
     if mean:
         assert cm <= 1e-6
+
+
+@pytest.mark.parametrize("dims", [[40, 40, 40], [48, 48, 48]])
+def test_mixed_ichol_chunked_fp32_solves_are_bitwise_the_level_launches(dims):
+    """The FP32 inner IC0 solves run as chunked persistent launches (FP32
+    records, FP64 row sums, one rounding per row) in sptrsv_mode 3: the mixed
+    solve's x, iteration count and residual history equal those with one
+    launch per level."""
+    print(f"[case] mixed ichol fp32 chunked dims={dims}", flush=True)
+    Ah = grid_laplacian(dims, 1e-8)
+    A = S.DeviceCSR.from_csr(Ah)
+    P = S.make_preconditioner(A, Ah, "ichol")
+    b = S.to_dev(rhs_normal(Ah.n, 43))
+    prev = S.set_option("sptrsv_mode", 0)
+    try:
+        r0 = S.pcg_mixed(A, b, P, tol=1e-10, maxit=5000)
+        S.set_option("sptrsv_mode", 3)
+        r3 = S.pcg_mixed(A, b, P, tol=1e-10, maxit=5000)
+    finally:
+        S.set_option("sptrsv_mode", prev)
+    print(f"[diag] mixed ichol {dims}: it {r3.iterations} vs {r0.iterations}, "
+          f"{r3.gpu_ms:.1f} ms vs {r0.gpu_ms:.1f} ms", flush=True)
+    assert r0.flag == 0 and (r3.iterations, r3.flag) == (r0.iterations, r0.flag)
+    assert np.array_equal(r3.history, r0.history)
+    assert np.array_equal(r3.x.cpu().numpy(), r0.x.cpu().numpy())
