@@ -338,11 +338,16 @@ def bench_energy(S, torch, A, b, P, tol: float, solves: int = 3) -> dict:
     """Energy to solution of the FP64 and mixed-precision Jacobi solves on the
     256^3 system: the GPU's energy accumulator read around `solves`
     back-to-back solves, the socket power sampled every 20 ms beside it (the
-    reference's time x mean-power figure), and the idle power over 1 s."""
+    reference's time x mean-power figure), and the idle power over 2 s after 3 s
+    of settling (right after load the board's power decays for a second or
+    more, and the sampled power, an average over the firmware's own window,
+    lags the accumulator there)."""
     from lap_amd.energy import EnergyMeter
     out = {}
+    torch.cuda.synchronize()
+    time.sleep(3.0)
     with EnergyMeter() as idle:
-        time.sleep(1.0)
+        time.sleep(2.0)
     if not idle.available:
         return idle.report()
     out["idle"] = idle.report()

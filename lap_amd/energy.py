@@ -13,6 +13,15 @@ socket power is sampled every `period` seconds on a thread beside it, so
 both the accumulator's energy and the reference's time x mean-power figure
 are reported.  When the AMD SMI library cannot be loaded or does not expose
 the counters, the meter reports itself unavailable (no fallback figure).
+
+The accumulator does not advance smoothly: the firmware adds to it in steps
+(tens to hundreds of milliseconds apart), so a difference of two reads taken
+at arbitrary instants over a window of a second or two can be off by a whole
+step, in either direction.  The sampling thread therefore also reads the
+accumulator, and the meter takes the mean power between the first and the
+last step it saw inside the region (energy between two update instants over
+the time between them); that power times the region's length is the energy.
+The plain two-read difference is kept beside it.
 """
 from __future__ import annotations
 
@@ -58,6 +67,10 @@ class EnergyMeter:
         self.period = period
         self.smi, self.h, self.bdf, self.error = _load_smi()
         self.samples: list[tuple[float, float]] = []
+        self.energy_samples: list[tuple[float, float]] = []
+        self.power_key = None
+        self.edge_watts = None
+        self.updates = 0
         self.joules = None
         self.seconds = None
         self.mean_watts = None
@@ -77,18 +90,32 @@ class EnergyMeter:
         for key in ("current_socket_power", "socket_power", "average_socket_power"):
             v = p.get(key)
             if isinstance(v, (int, float)) and v not in (0, 0xFFFF):
+                self.power_key = key
                 return float(v)
         return None
 
     def _sample(self):
         while not self._stop.is_set():
             try:
+                e = self._energy_j()
+                self.energy_samples.append((time.perf_counter(), e))
                 w = self._power_w()
                 if w is not None:
                     self.samples.append((time.perf_counter(), w))
             except Exception:
                 pass
             self._stop.wait(self.period)
+
+    def _edge_power(self):
+        """Mean power between the first and the last accumulator step seen
+        in the region, with the number of steps; (None, steps) when fewer
+        than two steps span at least a quarter of the region."""
+        es = [(self._t0, self._e0)] + self.energy_samples + [(self._t1, self._e1)]
+        steps = [(t, e) for (tp, ep), (t, e) in zip(es, es[1:]) if e != ep]
+        if len(steps) < 2 or steps[-1][0] - steps[0][0] < 0.25 * self.seconds:
+            return None, len(steps)
+        (ta, ea), (tb, eb) = steps[0], steps[-1]
+        return (eb - ea) / (tb - ta), len(steps)
 
     def __enter__(self):
         if not self.available:
@@ -102,12 +129,14 @@ class EnergyMeter:
     def __exit__(self, *exc):
         if not self.available:
             return False
-        t1 = time.perf_counter()
-        e1 = self._energy_j()
+        self._t1 = time.perf_counter()
+        self._e1 = self._energy_j()
         self._stop.set()
         self._thread.join()
-        self.seconds = t1 - self._t0
-        self.joules = e1 - self._e0
+        self.seconds = self._t1 - self._t0
+        self.edge_watts, self.updates = self._edge_power()
+        self.joules = (self.edge_watts * self.seconds if self.edge_watts is not None
+                       else self._e1 - self._e0)
         if self.samples:
             self.mean_watts = sum(w for _, w in self.samples) / len(self.samples)
         return False
@@ -118,11 +147,20 @@ class EnergyMeter:
         out = {
             "energy_j_per_solve": self.joules / solves,
             "mean_power_w_counter": self.joules / self.seconds,
+            "counter_method": "between_steps" if self.edge_watts is not None else "two_reads",
+            "counter_steps": self.updates,
+            "energy_j_per_solve_two_reads": (self._e1 - self._e0) / solves,
             "seconds": self.seconds, "solves": solves,
             "power_samples": len(self.samples), "gpu_bdf": self.bdf,
         }
+        if self.updates >= 2:
+            es = [t for (tp, ep), (t, e) in zip(self.energy_samples, self.energy_samples[1:])
+                  if e != ep]
+            if len(es) >= 2:
+                out["counter_step_interval_s"] = (es[-1] - es[0]) / (len(es) - 1)
         if self.mean_watts is not None:
             out["mean_power_w_sampled"] = self.mean_watts
+            out["power_field"] = self.power_key
             out["energy_j_per_solve_time_x_power"] = self.mean_watts * self.seconds / solves
             # the two sources should tell the same story; a counter that implies
             # a mean power far from the sampled one (or above the board's limit)
