@@ -317,6 +317,15 @@ struct TriPlan {
   // the longer ones: level l's long rows are order[level_long[l] ..
   // level_ptr[l + 1]).  A long row is solved by a whole workgroup.
   std::vector<int> level_long;  // host, nlevels
+  // Long rows split into segments of kTriSeg entries for the tree-summed
+  // level launches (option sptrsv_long 1): level l's segments are
+  // segd[level_seg[l] .. level_seg[l + 1]), int4 {position in the level,
+  // segment, segments of the row, slot}; a row's segments are consecutive
+  // and its slot is the index of its first one, which addresses the row's
+  // partial sums (seg_part[slot + segment], double) and ticket
+  // (seg_tick[slot], int, zero between solves).
+  std::vector<int> level_seg;  // host, nlevels + 1
+  DevBufPtr segd, seg_part, seg_tick;
   bool has_diag = true;        // every row holds its pivot
   // Chunked schedule (one persistent workgroup per chunk of consecutive rows
   // in solve order; built when every row holds at most 8 entries).  Chunk c
@@ -340,6 +349,8 @@ struct TriPlan {
 // Rows longer than this many entries (pivot included) are solved by a
 // workgroup each in the level launches.
 constexpr int kTriLongRow = 32;
+// Entries per workgroup of a tree-summed long row (a multiple of kBlock).
+constexpr int kTriSeg = 1024;
 const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower);
 // A triangle's rows in level order as fixed-width records (w = 4 or 8
 // entries, padded with column -1): ja[t*w + k], a[t*w + k] for the row at

@@ -37,8 +37,9 @@ namespace lap {
 
 namespace {
 
-// desc[t] = {row, first nonzero, end}: one load gives the row and its range
-// (the row pointers are not read in the solve).
+// desc[t] = {row, first nonzero, end, pivot}: one load gives the row, its
+// range and the position of its last diagonal entry (-1: none) (the row
+// pointers are not read in the solve).
 //
 // One launch per level.  The level's short rows (the first `cnt` records)
 // take one lane each; each of its `nlong` long rows (the records after them)
@@ -51,15 +52,92 @@ namespace {
 // values and vectors with FP64 products and sums (the mixed-precision inner
 // solve).
 constexpr int kTriChunk = 2048;
+constexpr int kTriFill = 12;  // products per thread per fill batch (2048 / 192 <= 12)
 
-// TREE (option sptrsv_long 1): a long row's products are summed by the
-// workgroup as a fixed-order tree instead -- deterministic, no serial chain,
-// but not the sequential order (tolerance parity, not bitwise).
+__device__ __forceinline__ double sf_load(const double *p) {
+  const unsigned long long v = __hip_atomic_load(
+      reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)v);
+}
+__device__ __forceinline__ float sf_load(const float *p) {
+  const unsigned v = __hip_atomic_load(reinterpret_cast<const unsigned *>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return __uint_as_float(v);
+}
+__device__ __forceinline__ void sf_store(double *p, double v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p),
+                     (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void sf_store(float *p, float v) {
+  __hip_atomic_store(reinterpret_cast<unsigned *>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// s - p[0] - p[1] - ... - p[m - 1], one term at a time in that order (the
+// sequential loop's result bit for bit).  p is LDS, 16-byte aligned, with at
+// least 12 readable elements past m (their values are never used).  Batches
+// of 12 terms alternate between two register sets and the next batch's loads
+// are issued before the current batch's subtractions, so the dependent chain
+// pays the FP64 add latency (~6.5 cycles) per term rather than an LDS round
+// trip per batch; at most 12 LDS loads are in flight (the counter holds 15).
+__device__ __forceinline__ double lds_seq_sub_ahead(double s, const double *p, int m) {
+  constexpr int H = 6;  // 16-byte loads per batch
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+  const int full = m / (2 * H);
+  double2 v[H], w[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) v[j] = q[j];
+  int b = 0;
+  // (the scheduling barriers keep the compiler from gathering both batches'
+  // loads in one place, which would expose a round trip again)
+  for (; b + 2 <= full; b += 2) {
+#pragma unroll
+    for (int j = 0; j < H; ++j) w[j] = q[(b + 1) * H + j];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      s -= v[j].x;
+      s -= v[j].y;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < H; ++j) v[j] = q[(b + 2) * H + j];  // ends <= m + 11
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      s -= w[j].x;
+      s -= w[j].y;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if (b < full) {
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      s -= v[j].x;
+      s -= v[j].y;
+    }
+    ++b;
+  }
+  for (int k = b * 2 * H; k < m; ++k) s -= p[k];
+  return s;
+}
+
+// TREE (option sptrsv_long 1): a long row's products are summed in a fixed
+// tree order instead -- deterministic, no serial chain, but not the
+// sequential order (tolerance parity, not bitwise).  The row is split into
+// segments of kTriSeg entries, one workgroup each (the launch's long-row
+// workgroups index segd: {level position, segment, segments of the row, the
+// row's slot}); each forms its segment's sum as a workgroup tree, and a row
+// of several segments is finished by the workgroup that arrives last on the
+// row's ticket: it adds the segments' sums in segment order (so the result
+// does not depend on which one that is) and resets the ticket for the next
+// solve.  Partial sums and the ticket are agent-scope, ordered by fences.
 template <class AT, class VT, bool TREE = false>
 __global__ __launch_bounds__(kBlock) void k_tri_level(
     int cnt, int nlong, const int4 *__restrict__ desc, const int *__restrict__ ja,
     const AT *__restrict__ a, const double *__restrict__ diag, const VT *x, VT *y,
-    const int *__restrict__ done) {
+    const int *__restrict__ done, const int4 *__restrict__ segd, double *part, int *tick) {
   if (done && *done) return;
   const int gshort = (cnt + kBlock - 1) / kBlock;
   if ((int)blockIdx.x < gshort) {
@@ -89,32 +167,51 @@ __global__ __launch_bounds__(kBlock) void k_tri_level(
   if (lr >= nlong) return;
   if (TREE) {
     __shared__ double red[kBlock / kWave];
-    __shared__ int tpos;
-    const int4 d = desc[cnt + lr];
+    __shared__ int last;
+    const int4 sd = segd[lr];
+    const int4 d = desc[sd.x];
     const int row = d.x;
-    if (threadIdx.x == 0) tpos = -1;
-    __syncthreads();
+    const int e0 = d.y + sd.y * kTriSeg, e1 = min(d.z, e0 + kTriSeg);
+    constexpr int B = kTriSeg / kBlock;
+    int c[B];
+    AT av[B];
+    VT yv[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int e = e0 + (int)threadIdx.x + j * kBlock;
+      c[j] = e < e1 ? ja[e] : row;
+      av[j] = e < e1 ? a[e] : AT(0);
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) yv[j] = c[j] != row ? y[c[j]] : VT(0);
     double s = 0.0;
-#pragma unroll 4
-    for (int e = d.y + (int)threadIdx.x; e < d.z; e += kBlock) {
-      const int c = ja[e];
-      if (c == row)
-        atomicMax(&tpos, e);
-      else
-        s += (double)a[e] * (double)y[c];
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      if (c[j] != row) s += (double)av[j] * (double)yv[j];
+    const double t = block_sum<kBlock>(s, red);  // valid in thread 0
+    auto finish = [&](double sum) {
+      const double piv = d.w >= 0 ? (double)a[d.w] : diag ? diag[row] : 0.0;
+      y[row] = (VT)(((double)x[row] - sum) / piv);
+    };
+    if (sd.z == 1) {
+      if (threadIdx.x == 0) finish(t);
+      return;
     }
-    const double t = block_sum<kBlock>(s, red);  // ends with a barrier: tpos is final
     if (threadIdx.x == 0) {
-      double piv = 0.0;
-      if (tpos >= 0)
-        piv = (double)a[tpos];
-      else if (diag)
-        piv = diag[row];
-      y[row] = (VT)(((double)x[row] - t) / piv);
+      sf_store(part + sd.w + sd.y, t);
+      __threadfence();
+      last = atomicAdd(tick + sd.w, 1) == sd.z - 1;
     }
+    __syncthreads();
+    if (threadIdx.x != 0 || !last) return;
+    __threadfence();
+    double sum = 0.0;
+    for (int j = 0; j < sd.z; ++j) sum += sf_load(part + sd.w + j);
+    __hip_atomic_store(tick + sd.w, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    finish(sum);
     return;
   }
-  __shared__ double prod[2][kTriChunk];
+  __shared__ __align__(16) double prod[2][kTriChunk + 16];  // + lds_seq_sub_ahead's read-ahead
   __shared__ int dpos;
   const int4 d = desc[cnt + lr];
   const int row = d.x;
@@ -122,26 +219,26 @@ __global__ __launch_bounds__(kBlock) void k_tri_level(
   if (threadIdx.x == 0) dpos = -1;
   __syncthreads();
   double acc = (double)x[row];
-  // threads t0 .. t0 + nt - 1 form chunk ch's products, loads issued in
-  // batches of four ahead of their use
+  // threads t0 .. t0 + nt - 1 form chunk ch's products, all of a thread's
+  // loads for the chunk issued together (one batch of kTriFill)
   auto fill = [&](int ch, int t0, int nt) {
     const int base = d.y + ch * kTriChunk;
     const int m = min(kTriChunk, d.z - base);
     double *buf = prod[ch & 1];
-    for (int k0 = (int)threadIdx.x - t0; k0 < m; k0 += 4 * nt) {
-      int c[4];
-      AT av[4];
-      VT yv[4];
+    for (int k0 = (int)threadIdx.x - t0; k0 < m; k0 += kTriFill * nt) {
+      int c[kTriFill];
+      AT av[kTriFill];
+      VT yv[kTriFill];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < kTriFill; ++j) {
         const int k = k0 + j * nt;
         c[j] = k < m ? ja[base + k] : row;
         av[j] = k < m ? a[base + k] : AT(0);
       }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) yv[j] = c[j] != row ? y[c[j]] : VT(0);
+      for (int j = 0; j < kTriFill; ++j) yv[j] = c[j] != row ? y[c[j]] : VT(0);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < kTriFill; ++j) {
         const int k = k0 + j * nt;
         if (k >= m) continue;
         if (c[j] == row) {
@@ -159,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_level(
     __syncthreads();  // chunk ch formed, chunk ch - 1 consumed
     if (threadIdx.x < kWave) {
       if (threadIdx.x == 0)
-        acc = lds_seq_sum<true>(acc, prod[ch & 1], 0, min(kTriChunk, d.z - d.y - ch * kTriChunk));
+        acc = lds_seq_sub_ahead(acc, prod[ch & 1], min(kTriChunk, d.z - d.y - ch * kTriChunk));
     } else if (ch + 1 < nch) {
       fill(ch + 1, kWave, kBlock - kWave);
     }
@@ -235,26 +332,6 @@ __global__ __launch_bounds__(kBlock) void k_tri_level_ell(
   }
   if (!in_row && diag) piv = diag[row];
   y[row] = acc / piv;
-}
-
-__device__ __forceinline__ double sf_load(const double *p) {
-  const unsigned long long v = __hip_atomic_load(
-      reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)v);
-}
-__device__ __forceinline__ float sf_load(const float *p) {
-  const unsigned v = __hip_atomic_load(reinterpret_cast<const unsigned *>(p), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-  return __uint_as_float(v);
-}
-__device__ __forceinline__ void sf_store(double *p, double v) {
-  __hip_atomic_store(reinterpret_cast<unsigned long long *>(p),
-                     (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void sf_store(float *p, float v) {
-  __hip_atomic_store(reinterpret_cast<unsigned *>(p), __float_as_uint(v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------- chunked mode
@@ -918,12 +995,33 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
   p->order = dev_upload(order);
   std::vector<int> desc((size_t)n * 4);
   for (int t = 0; t < n; ++t) {
-    desc[(size_t)t * 4] = order[(size_t)t];
-    desc[(size_t)t * 4 + 1] = ia[order[(size_t)t]];
-    desc[(size_t)t * 4 + 2] = ia[order[(size_t)t] + 1];
-    desc[(size_t)t * 4 + 3] = 0;
+    const int i = order[(size_t)t];
+    int piv = -1;  // the last diagonal entry, as the sequential loop keeps
+    for (int e = ia[i]; e < ia[i + 1]; ++e)
+      if (ja[e] == i) piv = e;
+    desc[(size_t)t * 4] = i;
+    desc[(size_t)t * 4 + 1] = ia[i];
+    desc[(size_t)t * 4 + 2] = ia[i + 1];
+    desc[(size_t)t * 4 + 3] = piv;
   }
   p->desc = dev_upload(desc);
+  // segments of the long rows (tree-summed level launches)
+  std::vector<int> segd;
+  p->level_seg.assign((size_t)nlev + 1, 0);
+  for (int l = 0; l < nlev; ++l) {
+    for (int t = p->level_long[l]; t < p->level_ptr[l + 1]; ++t) {
+      const int i = order[(size_t)t];
+      const int ns = (ia[i + 1] - ia[i] + kTriSeg - 1) / kTriSeg;
+      const int slot = (int)(segd.size() / 4);
+      for (int g = 0; g < ns; ++g)
+        segd.insert(segd.end(), {t - p->level_ptr[l], g, ns, slot});
+    }
+    p->level_seg[(size_t)l + 1] = (int)(segd.size() / 4);
+  }
+  const size_t nseg = segd.size() / 4;
+  p->segd = dev_upload(segd);
+  p->seg_part = dev_alloc(sizeof(double) * (nseg + 1));
+  p->seg_tick = dev_upload(std::vector<int>(nseg + 1, 0));
   tri_chunks_build(*p, ia, ja, level, order);
   return p;
 }
@@ -1004,6 +1102,33 @@ void tri_ch_pack32(const TriPlan &p, const int *ja, const float *a, TriChunks32 
   c.ok = true;
 }
 
+// One launch per level: a lane per short row, then a workgroup per long row
+// (sequential order) or per long-row segment (option sptrsv_long 1).
+template <class AT, class VT>
+static void tri_level_launch(const TriPlan &p, const int *ja, const AT *a, const double *diag,
+                             const VT *x, VT *y, const int *done) {
+  const int4 *desc = p.desc->as<int4>();
+  const bool tree = options().sptrsv_long == 1;
+  const int4 *segd = p.segd->as<int4>();
+  double *part = p.seg_part->as<double>();
+  int *tick = p.seg_tick->as<int>();
+  for (int l = 0; l < p.nlevels; ++l) {
+    const int cnt = p.level_long[l] - p.level_ptr[l];
+    const int nlong = tree ? p.level_seg[(size_t)l + 1] - p.level_seg[(size_t)l]
+                           : p.level_ptr[l + 1] - p.level_long[l];
+    const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock + nlong);
+    if (tree)
+      k_tri_level<AT, VT, true><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l],
+                                                            ja, a, diag, x, y, done,
+                                                            segd + p.level_seg[(size_t)l], part,
+                                                            tick);
+    else
+      k_tri_level<AT, VT><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l], ja, a,
+                                                      diag, x, y, done, nullptr, nullptr,
+                                                      nullptr);
+  }
+}
+
 // Which path tri_solve_launch takes.
 enum TriPath { TRI_CHUNKS, TRI_LEVELS_ELL, TRI_LEVELS };
 
@@ -1063,33 +1188,13 @@ void tri_solve_launch(const TriPlan &p, const int *ia, const int *ja, const doub
     }
     return;
   }
-  const int4 *desc = p.desc->as<int4>();
-  for (int l = 0; l < p.nlevels; ++l) {
-    const int cnt = p.level_long[l] - p.level_ptr[l], nlong = p.level_ptr[l + 1] - p.level_long[l];
-    const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock + nlong);
-    if (options().sptrsv_long == 1)
-      k_tri_level<double, double, true><<<g, kBlock, 0, stream()>>>(
-          cnt, nlong, desc + p.level_ptr[l], ja, a, diag, x, y, done);
-    else
-      k_tri_level<double, double><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l],
-                                                              ja, a, diag, x, y, done);
-  }
+  tri_level_launch<double, double>(p, ja, a, diag, x, y, done);
 }
 
 template <class VT>
 static void tri_mixed_launch(const TriPlan &p, const int *ja, const float *a, const VT *x, VT *y,
                              const int *done) {
-  const int4 *desc = p.desc->as<int4>();
-  for (int l = 0; l < p.nlevels; ++l) {
-    const int cnt = p.level_long[l] - p.level_ptr[l], nlong = p.level_ptr[l + 1] - p.level_long[l];
-    const unsigned g = (unsigned)((cnt + kBlock - 1) / kBlock + nlong);
-    if (options().sptrsv_long == 1)
-      k_tri_level<float, VT, true><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l],
-                                                               ja, a, nullptr, x, y, done);
-    else
-      k_tri_level<float, VT><<<g, kBlock, 0, stream()>>>(cnt, nlong, desc + p.level_ptr[l], ja,
-                                                         a, nullptr, x, y, done);
-  }
+  tri_level_launch<float, VT>(p, ja, a, nullptr, x, y, done);
 }
 
 static bool tri_chunks32_ready(const TriPlan &p, const TriChunks32 *c) {

@@ -301,11 +301,15 @@ def tree_long_rows():
 
 @pytest.mark.parametrize("name", sorted(LONG_ROWS))
 def test_trisolve_long_rows_tree_tolerance(orc, name, tree_long_rows):
-    """Option sptrsv_long 1 sums a long row's products as a workgroup tree:
-    not the sequential order, so compared within a tolerance -- each solve
-    within 1e-12 of the sequential solve relative to the solution's size,
-    short rows untouched; IC0 PCG on the power-law graph within one iteration
-    and 1e-10 of the oracle's solution."""
+    """Option sptrsv_long 1 sums a long row's products in a fixed tree order,
+    rows of more than 1024 entries split over several workgroups (the
+    power-law graph's 5417-entry rows and the star's 5999-entry hub: 6
+    segments, finished by the last to arrive on the row's ticket): not the
+    sequential order, so compared within a tolerance -- each solve within
+    1e-12 of the sequential solve relative to the solution's size, short rows
+    untouched; IC0 PCG on the power-law graph within one iteration and 1e-10
+    of the oracle's solution (many solves in a row: the tickets reset), and
+    the mixed-precision IC0 solves to the mixed solver's bars."""
     A = LONG_ROWS[name]()
     print(f"[case] tree long rows {name} n={A.n}", flush=True)
     rng = np.random.default_rng(5)
@@ -328,3 +332,19 @@ def test_trisolve_long_rows_tree_tolerance(orc, name, tree_long_rows):
               flush=True)
         assert res.flag == 0 and abs(res.iterations - ito) <= 1
         assert err <= 1e-10
+        # the mixed-precision inner solves take the same split rows with FP32
+        # values (FP32 or FP64 inner vectors); refined to the same tolerance,
+        # with the mixed solver's own bars: FP64 true residual within tol,
+        # solution within 1e3 tol of the oracle's
+        for mv in (0, 1):
+            prev = S.set_option("mixed_vectors", mv)
+            try:
+                m = S.pcg_mixed(D, dev(b), P, tol=1e-10, maxit=4000)
+            finally:
+                S.set_option("mixed_vectors", prev)
+            xm = host(m.x)
+            true = np.linalg.norm(b - orc.matvec(A, xm)) / np.linalg.norm(b)
+            merr = np.linalg.norm(xm - xo) / np.linalg.norm(xo)
+            print(f"[diag] tree mixed IC0 (vectors {mv}): it {m.iterations}, true rel res "
+                  f"{true:.2e}, err {merr:.2e}", flush=True)
+            assert m.flag == 0 and true < 1e-10 * 1.01 and merr <= 1e3 * 1e-10
