@@ -11,6 +11,13 @@ adjoint PCG L(w) lambda = dl/dx with the forward's preconditioner and the
 fused edge kernel dl/dw_e = -(lambda_s - lambda_d)(x_s - x_d); dl/db =
 lambda (lap_pcg_adjoint).  No reference counterpart (the reference solves
 one unweighted Laplacian and has no derivative).
+
+Node order: by default the nodes are renumbered internally by reverse
+Cuthill-McKee, so that rows sharing columns sit together and the SpMV's
+gathers of x stay in cache (a random geometric graph in the caller's random
+order otherwise gathers from all of x: 2.8x slower per iteration at 1M
+nodes).  Vectors cross the interface in the caller's order; edges keep
+theirs.
 """
 from __future__ import annotations
 
@@ -30,16 +37,55 @@ def _idx(v) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(v, dtype=np.int32)).to(DEV)
 
 
-class GraphLaplacian:
-    """CSR pattern of L(w) = D(w) - W(w) + shift*I for fixed edges."""
+def _host(v) -> np.ndarray:
+    if isinstance(v, torch.Tensor):
+        return v.detach().cpu().numpy().astype(np.int64)
+    return np.asarray(v, dtype=np.int64)
 
-    def __init__(self, n: int, src, dst, shift: float = 0.0):
+
+def rcm_order(n: int, src: np.ndarray, dst: np.ndarray) -> np.ndarray:
+    """perm[k] = the node placed at internal row k: reverse Cuthill-McKee of
+    the undirected graph (scipy.sparse.csgraph; 0.3 s at 1M nodes, 5M
+    edges)."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    M = sp.coo_matrix((np.ones(len(src), np.int8), (src, dst)), shape=(n, n)).tocsr()
+    return reverse_cuthill_mckee((M + M.T).tocsr(), symmetric_mode=True).astype(np.int64)
+
+
+class GraphLaplacian:
+    """CSR pattern of L(w) = D(w) - W(w) + shift*I for fixed edges.
+
+    order "rcm" (default) numbers the nodes internally by reverse
+    Cuthill-McKee; "natural" keeps the caller's numbering.  The pattern
+    (ia, ja, epos, dpos), assemble() and preconditioner() are in the internal
+    order: internal row k is the caller's node perm[k] (perm is None for the
+    natural order).  laplacian_solve takes b and returns x (and dl/db) in
+    the caller's order; the edge order (w, dl/dw) is never changed.  src and
+    dst are the caller's; isrc and idst the same edges in internal numbers."""
+
+    def __init__(self, n: int, src, dst, shift: float = 0.0, order: str = "rcm"):
+        if order not in ("rcm", "natural"):
+            raise L.LapError(f"order must be 'rcm' or 'natural', not {order!r}")
         self.n = int(n)
+        self.order = order
         self.src, self.dst = _idx(src), _idx(dst)
         self.m = int(self.src.shape[0])
         if self.dst.shape[0] != self.m:
             raise L.LapError("src and dst differ in length")
         self.shift = float(shift)
+        self.perm = self.iperm = None
+        self.isrc, self.idst = self.src, self.dst
+        if order == "rcm" and self.m:
+            hs, hd = _host(src), _host(dst)
+            if hs.min() >= 0 and hd.min() >= 0 and max(hs.max(), hd.max()) < self.n:
+                perm = rcm_order(self.n, hs, hd)
+                iperm = np.empty(self.n, np.int64)
+                iperm[perm] = np.arange(self.n)
+                self.perm = torch.from_numpy(perm).to(DEV)
+                self.iperm = torch.from_numpy(iperm).to(DEV)
+                self.isrc, self.idst = _idx(iperm[hs]), _idx(iperm[hd])
+            # (out-of-range nodes: left to lap_graph_pattern to reject)
         nnz = self.n + 2 * self.m
         i32 = dict(dtype=torch.int32, device=DEV)
         self.ia = torch.empty(self.n + 1, **i32)
@@ -47,7 +93,7 @@ class GraphLaplacian:
         self.epos = torch.empty(max(2 * self.m, 1), **i32)
         self.dpos = torch.empty(self.n, **i32)
         _sync()
-        st = L.load().lap_graph_pattern(self.n, self.m, ptr(self.src), ptr(self.dst),
+        st = L.load().lap_graph_pattern(self.n, self.m, ptr(self.isrc), ptr(self.idst),
                                         ptr(self.ia), ptr(self.ja), ptr(self.epos),
                                         ptr(self.dpos))
         if st != 0:
@@ -59,6 +105,14 @@ class GraphLaplacian:
     @property
     def nnz(self) -> int:
         return self.n + 2 * self.m
+
+    def to_internal(self, v: torch.Tensor) -> torch.Tensor:
+        """A node vector in the caller's order -> internal order."""
+        return v if self.perm is None else v[self.perm]
+
+    def to_caller(self, v: torch.Tensor) -> torch.Tensor:
+        """A node vector in internal order -> the caller's order."""
+        return v if self.iperm is None else v[self.iperm]
 
     def assemble(self, w: torch.Tensor) -> tuple[DeviceCSR, torch.Tensor]:
         """(L(w) on the device, 1/diag(L(w)))."""
@@ -110,34 +164,34 @@ class _LaplacianSolve(torch.autograd.Function):
         from .solver import pcg
         A, d_r = G.assemble(w)
         P = G.preconditioner(A, d_r, prec)
-        bb = b.detach().to(device=DEV, dtype=torch.float64).contiguous()
+        bb = G.to_internal(b.detach().to(device=DEV, dtype=torch.float64)).contiguous()
         res = pcg(A, bb, P, tol=tol, maxit=maxit)
         G.last_forward = SolveStats(res.iterations, res.flag, res.res0, res.true_res, res.gpu_ms)
         if res.flag != 0:
             raise L.LapError(f"forward solve stopped with flag {res.flag}")
         ctx.G, ctx.A, ctx.P, ctx.tol, ctx.maxit = G, A, P, tol, maxit
-        ctx.save_for_backward(res.x)
-        return res.x
+        ctx.save_for_backward(res.x)  # internal order
+        return G.to_caller(res.x)
 
     @staticmethod
     def backward(ctx, gx):
         (x,) = ctx.saved_tensors
         G, A, P = ctx.G, ctx.A, ctx.P
-        gx = gx.detach().to(dtype=torch.float64).contiguous()
+        gx = G.to_internal(gx.detach().to(dtype=torch.float64)).contiguous()
         lam = torch.empty(G.n, dtype=torch.float64, device=DEV)
         gw = torch.empty(G.m, dtype=torch.float64, device=DEV) if ctx.needs_input_grad[0] else None
         info = L.SolveInfo()
         _sync()
         st = L.load().lap_pcg_adjoint(A.n, A.nnz, ptr(A.ia), ptr(A.ja), ptr(A.a),
                                       C.byref(P.pdata), ptr(gx), ptr(lam), ctx.tol, ctx.maxit,
-                                      G.m, ptr(G.src), ptr(G.dst), ptr(x), ptr(gw), None,
+                                      G.m, ptr(G.isrc), ptr(G.idst), ptr(x), ptr(gw), None,
                                       C.byref(info))
         G.last_adjoint = SolveStats(info.iterations, info.flag, info.res0, info.true_res,
                                     info.gpu_ms)
         if st != 0:
             L.check()
             raise L.LapError(f"lap_pcg_adjoint returned {st}")
-        return gw, (lam if ctx.needs_input_grad[1] else None), None, None, None, None
+        return gw, (G.to_caller(lam) if ctx.needs_input_grad[1] else None), None, None, None, None
 
 
 def laplacian_solve(G: GraphLaplacian, w: torch.Tensor, b: torch.Tensor,

@@ -11,7 +11,10 @@ Bars:
     and with central finite differences of l (evaluated with this solver)
     along random directions and for the five largest edge gradients to 1e-6
     relative, for line_jacobi and IC0 preconditioners;
-  * torch.autograd.gradcheck passes on a 40-node graph.
+  * torch.autograd.gradcheck passes on a 40-node graph;
+  * the default node order (reverse Cuthill-McKee inside the solver) is the
+    natural pattern renumbered, with a smaller bandwidth, and solves to the
+    same x; the gradient tests run in that order.
 """
 import numpy as np
 import pytest
@@ -44,7 +47,7 @@ def case():
 
 def test_pattern_and_values_bitwise(orc):
     g = small_graph(500, 2)
-    G = GR.GraphLaplacian(g.n, g.src, g.dst, SHIFT)
+    G = GR.GraphLaplacian(g.n, g.src, g.dst, SHIFT, order="natural")
     ia, ja, epos, dpos = pattern_maps(g)
     assert np.array_equal(G.ia.cpu().numpy(), ia) and np.array_equal(G.ja.cpu().numpy(), ja)
     assert np.array_equal(G.epos.cpu().numpy(), epos)
@@ -53,6 +56,40 @@ def test_pattern_and_values_bitwise(orc):
     a = orc.graph_values(g.n, ia, ja, epos, dpos, g.w, SHIFT)
     assert np.array_equal(A.a.cpu().numpy(), a)
     assert np.array_equal(d_r.cpu().numpy(), 1.0 / a[dpos])
+
+
+def test_rcm_pattern_is_the_permuted_natural_one(orc):
+    """order "rcm": internal row k is node perm[k]; the pattern and the edge
+    values equal the natural ones with rows and columns renumbered (the
+    degrees to rounding: summed in another order), and a solve returns the
+    caller's order (same x as the natural order to 1e-10)."""
+    g = random_geometric_graph(3000, 4)
+    Gn = GR.GraphLaplacian(g.n, g.src, g.dst, SHIFT, order="natural")
+    Gr = GR.GraphLaplacian(g.n, g.src, g.dst, SHIFT)
+    perm = Gr.perm.cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(g.n))
+    w = torch.from_numpy(g.w).to(DEV)
+    An, _ = Gn.assemble(w)
+    Ar, _ = Gr.assemble(w)
+    import scipy.sparse as sp
+    Mn = sp.csr_matrix((An.a.cpu().numpy(), An.ja.cpu().numpy(), An.ia.cpu().numpy()))
+    Mr = sp.csr_matrix((Ar.a.cpu().numpy(), Ar.ja.cpu().numpy(), Ar.ia.cpu().numpy()))
+    P = Mn[perm][:, perm].tocsr()
+    P.sort_indices()
+    assert np.array_equal(P.indptr, Mr.indptr) and np.array_equal(P.indices, Mr.indices)
+    off = P.indices != np.repeat(np.arange(g.n), np.diff(P.indptr))
+    assert np.array_equal(P.data[off], Mr.data[off])  # the edge weights, moved
+    # the diagonals are the degrees summed in the rows' (renumbered) column
+    # order: equal to rounding
+    np.testing.assert_allclose(P.data[~off], Mr.data[~off], rtol=1e-14, atol=0)
+    bw = Mr.indices - np.repeat(np.arange(g.n), np.diff(Mr.indptr))
+    bn = Mn.indices - np.repeat(np.arange(g.n), np.diff(Mn.indptr))
+    print(f"[diag] rcm bandwidth {np.abs(bw).max()} vs natural {np.abs(bn).max()}", flush=True)
+    assert np.abs(bw).max() < np.abs(bn).max()
+    b = torch.from_numpy(np.random.default_rng(2).standard_normal(g.n)).to(DEV)
+    xn = GR.laplacian_solve(Gn, w, b, tol=1e-12).cpu().numpy()
+    xr = GR.laplacian_solve(Gr, w, b, tol=1e-12).cpu().numpy()
+    assert rel(xr, xn) <= 1e-10
 
 
 def test_edge_kernel_bitwise(orc):
