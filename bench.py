@@ -374,7 +374,7 @@ def bench_energy(S, torch, A, b, P, tol: float, solves: int = 3) -> dict:
 
 
 def bench_power_law(S, torch, np, tol: float, n: int = 1 << 22, reps: int = 2,
-                    kinds=("line_jacobi", "ichol")) -> dict:
+                    kinds=("line_jacobi", "ichol"), maxit: int = 25000) -> dict:
     """The mixed-precision workload on a power-law graph: weighted Chung-Lu
     Laplacian, n = 2^22, degree exponent 2.1, mean degree 16, degrees capped
     at 20000, lognormal(0, 1) weights, diagonal (1 + 1e-6) x the weight sum;
@@ -383,7 +383,9 @@ def bench_power_law(S, torch, np, tol: float, n: int = 1 << 22, reps: int = 2,
     refinement to `tol`) with FP64 inner vectors (mixed_vectors 1) and, for
     Jacobi, FP32 inner vectors (mixed_vectors 0); Jacobi and IC0.  Also the
     fused SpMV's rate on this matrix (gather-bound: its columns are random
-    beyond the hub prefix)."""
+    beyond the hub prefix).  maxit below the solves' iteration counts
+    (profiling runs) stops every solve there: the times are then not solve
+    times, and the record says so."""
     from lap_amd.problems import chung_lu, rhs_normal
 
     t0 = time.perf_counter()
@@ -393,6 +395,8 @@ def bench_power_law(S, torch, np, tol: float, n: int = 1 << 22, reps: int = 2,
                         f"12345 mean-subtracted; PCG to rel-res {tol:g}"),
            "n": n, "nnz": Ah.nnz, "max_row": int(np.diff(Ah.ia).max()),
            "build_s": time.perf_counter() - t0}
+    if maxit < 25000:
+        out["maxit_capped"] = maxit
     A = S.DeviceCSR.from_csr(Ah)
     b = S.to_dev(rhs_normal(n, 12345))
     x = torch.randn(n, dtype=torch.float64, device="cuda")
@@ -418,7 +422,7 @@ def bench_power_law(S, torch, np, tol: float, n: int = 1 << 22, reps: int = 2,
             P = S.make_preconditioner(A, Ah, kind)
             torch.cuda.synchronize()
             rec = {"setup_s": time.perf_counter() - t}
-            fp64 = lambda: S.pcg(A, b, P, tol=tol, maxit=25000)
+            fp64 = lambda: S.pcg(A, b, P, tol=tol, maxit=maxit)
             if count > 1:
                 fp64()
             res, wall = timed(fp64, count)
@@ -428,7 +432,7 @@ def bench_power_law(S, torch, np, tol: float, n: int = 1 << 22, reps: int = 2,
             log(f"power_law {kind} fp64: {d.iterations} iterations {wall:.3f} s")
             for mv in ((1, 0) if kind == "line_jacobi" else (1,)):
                 S.set_option("mixed_vectors", mv)
-                mixed = lambda: S.pcg_mixed(A, b, P, tol=tol, maxit=25000)
+                mixed = lambda: S.pcg_mixed(A, b, P, tol=tol, maxit=maxit)
                 if count > 1:
                     mixed()
                 res, mwall = timed(mixed, count)
@@ -449,7 +453,7 @@ def bench_power_law(S, torch, np, tol: float, n: int = 1 << 22, reps: int = 2,
                     res, twall = timed(fp64, 1)
                     t64 = res[-1]
                     S.set_option("mixed_vectors", 1)
-                    res, tmwall = timed(lambda: S.pcg_mixed(A, b, P, tol=tol, maxit=25000), 1)
+                    res, tmwall = timed(lambda: S.pcg_mixed(A, b, P, tol=tol, maxit=maxit), 1)
                     tm = res[-1]
                 finally:
                     S.set_option("sptrsv_long", prev_long)
@@ -801,6 +805,8 @@ def main() -> int:
                          "power_law, sweeps, block, batched, adjoint, input); default all")
     ap.add_argument("--power-law-kinds", default="line_jacobi,ichol",
                     help="preconditioners of the power_law extra")
+    ap.add_argument("--power-law-maxit", type=int, default=25000,
+                    help="iteration cap of the power_law solves (profiling runs only)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the kernels the HBM-traffic counters are read on: a 1 GiB "
                          "device copy (calibration) and the fused SpMV, then exit")
@@ -928,7 +934,8 @@ def main() -> int:
                                                       args.spmv_reps)),
                          ("energy", lambda: bench_energy(S, torch, A, b, P, args.tol)),
                          ("power_law", lambda: bench_power_law(
-                             S, torch, np, args.tol, kinds=args.power_law_kinds.split(","))),
+                             S, torch, np, args.tol, kinds=args.power_law_kinds.split(","),
+                             maxit=args.power_law_maxit)),
                          ("sweeps", lambda: bench_sweeps(S, torch, A, Ah, b, args.tol)),
                          ("block", lambda: bench_block(S, torch, np)),
                          ("batched", lambda: bench_batched(S, torch, np)),
