@@ -323,12 +323,13 @@ double lap_pcg_spmv_timed(int n, int nnz, const int *ia, const int *ja, const do
  * "spmv_rows" (rows per SpMV block), "spmv_walk" (row-block order of the
  * persistent SpMV grid: 0 cyclic, 3 groups of four consecutive blocks per
  * XCD), "spmv_wg_per_cu" (its workgroups per CU), "vec_grid" (workgroups
- * of the vector kernels), "graph_iters" (iterations per HIP graph launch),
- * "use_graph", "spmm_wg_per_cu" (block CG SpMM workgroups per CU),
- * "sptrsv_mode" (triangular solves: 0 one launch per level, 3, the
- * default, one persistent launch over chunks of rows with point-to-point
- * waits), "sptrsv_long" (1: rows of more than 32 entries summed as a
- * workgroup tree), "fuse_sweeps", "spmv_format" (0 CSR, 1 value-indexed,
+ * of the vector kernels, 1..4096), "graph_iters" (iterations per HIP graph
+ * launch, 1..4096), "use_graph" (0 or 1), "spmm_wg_per_cu" (block CG SpMM
+ * workgroups per CU), "sptrsv_mode" (triangular solves: 0 one launch per
+ * level, 3, the default, one persistent launch over chunks of rows with
+ * point-to-point waits), "sptrsv_long" (1: rows of more than 32 entries
+ * summed in a fixed tree order, rows of more than 1024 entries over several
+ * workgroups), "fuse_sweeps", "spmv_format" (0 CSR, 1 value-indexed,
  * 2 delta-indexed), "mixed_vectors" (lap_pcg_mixed inner vectors: 0 FP32,
  * 1 FP64).  Results do not depend on them except vec_grid, spmv_walk and
  * spmv_wg_per_cu (the dot-product summation order), sptrsv_long and

@@ -201,6 +201,14 @@ int lap_set_option(const char *name, int value) {
     set_error(LAP_ERR_ARG, "fuse_sweeps must be 0 or 1");
     return *slot;
   }
+  if (!strcmp(name, "graph_iters") && (value < 1 || value > 4096)) {
+    set_error(LAP_ERR_ARG, "graph_iters must be in [1, 4096]");
+    return *slot;
+  }
+  if (!strcmp(name, "use_graph") && (value < 0 || value > 1)) {
+    set_error(LAP_ERR_ARG, "use_graph must be 0 or 1");
+    return *slot;
+  }
   if (!strcmp(name, "spmv_rows") && (value < 1 || value > 2 * kBlock)) {
     set_error(LAP_ERR_ARG, "spmv_rows must be in [1, %d]", 2 * kBlock);
     return *slot;

@@ -20,6 +20,9 @@ OPTIONS = {
     "sptrsv_long": ([0, 1], -1, 0),
     "spmm_wg_per_cu": ([1, 16], 17, None),
     "spmv_wg_per_cu": ([1, 16], 0, 8),
+    "vec_grid": ([1, 4096], 4097, 768),
+    "graph_iters": ([1, 4096], 0, 16),
+    "use_graph": ([0, 1], 2, 1),
 }
 
 
