@@ -263,7 +263,12 @@ def test_prec_nine_point_bitwise(orc, kind, tri_mode):
     test_prec_bitwise(orc, nine_point(90, 70), kind)
 
 
-LONG_ROWS = {"power_law": lambda: chung_lu(16384, seed=7), "star": lambda: star_graph(6000)}
+LONG_ROWS = {"power_law": lambda: chung_lu(16384, seed=7), "star": lambda: star_graph(6000),
+             # hub rows at the long-row boundaries: 1024 entries (one tree segment),
+             # 1025 (two, the second holding one entry), 2049 (three; and two LDS
+             # chunks of the sequential path, the second holding one entry)
+             "star_1024": lambda: star_graph(1024), "star_1025": lambda: star_graph(1025),
+             "star_2049": lambda: star_graph(2049)}
 
 
 @pytest.mark.parametrize("name", sorted(LONG_ROWS))
