@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <tuple>
+
 #include "lap_core.hpp"
 
 namespace lap {
@@ -183,10 +185,11 @@ const IcPlan *ic_plan_find(const double *vals) {
   return it == g_ic.end() ? nullptr : it->second.plan.get();
 }
 
-// Level analysis for a pattern held on the host (shares the solve kernels'
-// TriPlan layout).
-std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
-                                          const std::vector<int> &ja, bool lower);
+// Level analyses of a lower and an upper pattern held on the host (shares
+// the solve kernels' TriPlan layout), the two at once.
+std::pair<std::unique_ptr<TriPlan>, std::unique_ptr<TriPlan>> tri_analyse_pair(
+    int n, const std::vector<int> &lia, const std::vector<int> &lja,
+    const std::vector<int> &uia, const std::vector<int> &uja);
 
 const IcPlan *ic_factor(int n, int nnzA, const int *ia, const int *ja, double *a) {
   if (n < 0 || nnzA < 0) {
@@ -255,8 +258,7 @@ const IcPlan *ic_factor(int n, int nnzA, const int *ia, const int *ja, double *a
   P.uval = dev_alloc(sizeof(double) * (size_t)std::max(nnzL, 1));
   auto d_lpos = dev_upload(lpos);
   auto d_tperm = dev_upload(tperm);
-  ent.lplan = tri_analyse_host(n, lia, lja, true);
-  ent.uplan = tri_analyse_host(n, uia, uja, false);
+  std::tie(ent.lplan, ent.uplan) = tri_analyse_pair(n, lia, lja, uia, uja);
   if (!ent.lplan || !ent.uplan) return nullptr;
   P.lplan = ent.lplan.get();
   P.uplan = ent.uplan.get();

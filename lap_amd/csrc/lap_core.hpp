@@ -352,6 +352,9 @@ constexpr int kTriLongRow = 32;
 // Entries per workgroup of a tree-summed long row (a multiple of kBlock).
 constexpr int kTriSeg = 1024;
 const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower);
+// Both plans of a lower/upper pair (GS_std's splitting), analysed at once.
+void tri_plan_pair(int n, int nnzL, const int *lia, const int *lja, int nnzU, const int *uia,
+                   const int *uja);
 // A triangle's rows in level order as fixed-width records (w = 4 or 8
 // entries, padded with column -1): ja[t*w + k], a[t*w + k] for the row at
 // level-order position t.  Built for one set of values; w == 0: not built.

@@ -621,6 +621,7 @@ static bool prec_setup(PrecOps &o, int n, int nnzA, const int *ia, const int *ja
                                  {pd->la, "prec_data->la"}, {pd->uia, "prec_data->uia"},
                                  {pd->uja, "prec_data->uja"}, {pd->ua, "prec_data->ua"}}))
       return false;
+    tri_plan_pair(n, pd->lnnz, pd->lia, pd->lja, pd->unnz, pd->uia, pd->uja);
     o.Lt = tri_plan(n, pd->lnnz, pd->lia, pd->lja, true);
     o.Ut = tri_plan(n, pd->unnz, pd->uia, pd->uja, false);
     if (!o.Lt || !o.Ut) return false;

@@ -696,6 +696,17 @@ void forget_tri_plans(const void *p) {
 // against 255 for z-slabs).
 constexpr int kHopCost = 6;
 
+// f(t0, t1) over contiguous ranges covering [0, n), one per host thread.
+template <class F>
+static void host_ranges(int n, F f) {
+  const int T = std::max(1, std::min(host_threads(), n / 65536 + 1));
+  std::vector<std::thread> th;
+  for (int k = 1; k < T; ++k)
+    th.emplace_back(f, (int)((long)n * k / T), (int)((long)n * (k + 1) / T));
+  f(0, (int)((long)n / T));
+  for (std::thread &t : th) t.join();
+}
+
 struct ChunkSched {
   int G = 0;
   std::vector<int> cbase, crow, cmap;  // rows in (chunk, level, row) order
@@ -896,19 +907,21 @@ static void tri_chunks_build(TriPlan &p, const std::vector<int> &ia, const std::
         lpos[(size_t)best.crow[(size_t)t]] = t - best.sptr[(size_t)k];
       }
   std::vector<int> code((size_t)n * W, -1);
-  for (int t = 0; t < n; ++t) {
-    const int i = best.crow[(size_t)t];
-    int k = 0;
-    for (int e = ia[i]; e < ia[i + 1]; ++e) {
-      const int d = ja[e];
-      if (d == i) continue;  // the pivot is kept apart
-      int cd = d;
-      if (chunk_of[(size_t)d] == chunk_of[(size_t)i] &&
-          lstep[(size_t)i] - lstep[(size_t)d] < kChRing)
-        cd = -(2 + (lstep[(size_t)d] % kChRing) * kChStepRows + lpos[(size_t)d]);
-      code[(size_t)t * W + k++] = cd;
+  host_ranges(n, [&](int t0, int t1) {  // records are independent: contiguous ranges per thread
+    for (int t = t0; t < t1; ++t) {
+      const int i = best.crow[(size_t)t];
+      int k = 0;
+      for (int e = ia[i]; e < ia[i + 1]; ++e) {
+        const int d = ja[e];
+        if (d == i) continue;  // the pivot is kept apart
+        int cd = d;
+        if (chunk_of[(size_t)d] == chunk_of[(size_t)i] &&
+            lstep[(size_t)i] - lstep[(size_t)d] < kChRing)
+          cd = -(2 + (lstep[(size_t)d] % kChRing) * kChStepRows + lpos[(size_t)d]);
+        code[(size_t)t * W + k++] = cd;
+      }
     }
-  }
+  });
   // the progress counters and the error flag, allocated here (never inside
   // a graph capture of the solve)
   if (!ch_progress() || !sf_state()) return;
@@ -1026,10 +1039,9 @@ std::unique_ptr<TriPlan> tri_analyse_host(int n, const std::vector<int> &ia,
   return p;
 }
 
-const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower) {
-  Key key{ia, ja, n, nnz, lower};
-  auto it = g_plans.find(key);
-  if (it != g_plans.end()) return it->second.get();
+// The analysis of a device pattern (not cached).
+static std::unique_ptr<TriPlan> tri_plan_make(int n, int nnz, const int *ia, const int *ja,
+                                              bool lower) {
   if (n < 0 || nnz < 0) {
     set_error(LAP_ERR_ARG, "triangular solve: negative size");
     return nullptr;
@@ -1041,11 +1053,50 @@ const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower
     return nullptr;
   }
   std::vector<int> h_ja = dev_download(ja, (size_t)total);
-  auto plan = tri_analyse_host(n, h_ia, h_ja, lower);
+  return tri_analyse_host(n, h_ia, h_ja, lower);
+}
+
+const TriPlan *tri_plan(int n, int nnz, const int *ia, const int *ja, bool lower) {
+  Key key{ia, ja, n, nnz, lower};
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) return it->second.get();
+  auto plan = tri_plan_make(n, nnz, ia, ja, lower);
   if (!plan) return nullptr;
   TriPlan *raw = plan.get();
   g_plans[key] = std::move(plan);
   return raw;
+}
+
+std::pair<std::unique_ptr<TriPlan>, std::unique_ptr<TriPlan>> tri_analyse_pair(
+    int n, const std::vector<int> &lia, const std::vector<int> &lja,
+    const std::vector<int> &uia, const std::vector<int> &uja) {
+  // the shared device buffers are allocated lazily: before the threads
+  ch_progress();
+  sf_state();
+  std::unique_ptr<TriPlan> up;
+  std::thread tu([&] { up = tri_analyse_host(n, uia, uja, false); });
+  auto lp = tri_analyse_host(n, lia, lja, true);
+  tu.join();
+  return {std::move(lp), std::move(up)};
+}
+
+void tri_plan_pair(int n, int nnzL, const int *lia, const int *lja, int nnzU, const int *uia,
+                   const int *uja) {
+  const Key kl{lia, lja, n, nnzL, true}, ku{uia, uja, n, nnzU, false};
+  const bool need_l = !g_plans.count(kl), need_u = !g_plans.count(ku);
+  if (!need_l || !need_u) {  // one (or none) to build
+    tri_plan(n, nnzL, lia, lja, true);
+    tri_plan(n, nnzU, uia, uja, false);
+    return;
+  }
+  ch_progress();
+  sf_state();
+  std::unique_ptr<TriPlan> up;
+  std::thread tu([&] { up = tri_plan_make(n, nnzU, uia, uja, false); });
+  auto lp = tri_plan_make(n, nnzL, lia, lja, true);
+  tu.join();
+  if (lp) g_plans[kl] = std::move(lp);
+  if (up) g_plans[ku] = std::move(up);
 }
 
 int tri_error() {
@@ -1273,8 +1324,7 @@ void initialize_and_analyze_L_and_U_solve(const int n, const int nnzL, int *lia,
     return;
   g_plans.erase(Key{lia, lja, n, nnzL, true});  // explicit analysis rebuilds
   g_plans.erase(Key{uia, uja, n, nnzU, false});
-  tri_plan(n, nnzL, lia, lja, true);
-  tri_plan(n, nnzU, uia, uja, false);
+  tri_plan_pair(n, nnzL, lia, lja, nnzU, uia, uja);
 }
 
 void hip_lower_triangular_solve(const int n, const int nnzL, const int *lia, const int *lja,
