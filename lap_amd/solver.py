@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import lib as L
-from .problems import CSR, augment_lower, augment_upper, diagonal, split
+from .problems import CSR, diagonal
 
 DEV = "cuda"
 
@@ -134,13 +134,13 @@ class Preconditioner:
 def make_preconditioner(A: DeviceCSR, host: CSR, kind: str, m: int = 1, k: int = 1,
                         ) -> Preconditioner:
     """Mirrors cg_driver2.c's GPU-path setup (:95-144, :219-252, :328-338)."""
+    from .io import split as device_split  # (io imports this module)
     lib = L.load()
     n = A.n
-    Ls, d, Us = split(host)
-    if kind == "GS_std":
-        Ls, Us = augment_lower(host), augment_upper(host)
-    Ld, Ud = DeviceCSR.from_csr(Ls), DeviceCSR.from_csr(Us)
-    d_dev = to_dev(d)
+    # the strict triangles and the diagonal, or for GS_std L+D and D+U, split
+    # on the device (lap_csr_split: the same arrays as a host split of `host`,
+    # which describes the same matrix)
+    Ld, d_dev, Ud = device_split(A, augmented=(kind == "GS_std"))
     d_r = torch.empty(n, dtype=torch.float64, device=DEV)
     _sync()
     lib.hip_vector_reciprocal(n, ptr(d_dev), ptr(d_r))
