@@ -112,6 +112,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Streamed vector elements: n-long vectors that a kernel reads or writes
+// once, far larger than the L2s.  Non-temporal loads and stores keep them
+// from displacing lines that are reused (the SpMV's gathers of p).  Values
+// are unchanged; only the cache policy differs.
+__device__ __forceinline__ double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+
 // s (+ or -) p[k0] (+ or -) p[k0 + 1] ... (+ or -) p[k1 - 1], one term at a
 // time in that order: the sequential loop's result bit for bit.  The loads
 // (p is LDS) are issued eight at a time ahead of the dependent adds, so the
