@@ -116,8 +116,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 // once, far larger than the L2s.  Non-temporal loads and stores keep them
 // from displacing lines that are reused (the SpMV's gathers of p).  Values
 // are unchanged; only the cache policy differs.
-__device__ __forceinline__ double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+template <class T>
+__device__ __forceinline__ T ld_nt(const T *p) { return __builtin_nontemporal_load(p); }
+template <class T>
+__device__ __forceinline__ void st_nt(T *p, T v) { __builtin_nontemporal_store(v, p); }
 
 // s (+ or -) p[k0] (+ or -) p[k0 + 1] ... (+ or -) p[k1 - 1], one term at a
 // time in that order: the sequential loop's result bit for bit.  The loads

@@ -159,7 +159,7 @@ struct PqMxEpi {
   __device__ double init(int row) const { return diag[row] * (double)p[row]; }
   __device__ void store(int row, double s) {
     const VT qf = (VT)s;
-    q[row] = qf;
+    st_nt(q + row, qf);  // q is read once, by the next kernel
     acc += (double)p[row] * (double)qf;
   }
 };
@@ -316,13 +316,14 @@ __global__ __launch_bounds__(BS) void k_mx_update(int n, VT *__restrict__ x32,
   const double alpha = ctl->rho / pq;
   double rz = 0.0, rr = 0.0;
   const size_t stride = (size_t)gridDim.x * BS;
+  // streamed vectors: non-temporal loads and stores (values unchanged)
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
-    x32[i] = (VT)((double)x32[i] + alpha * (double)p[i]);
-    const VT rf = (VT)((double)r32[i] - alpha * (double)q[i]);
-    r32[i] = rf;
+    st_nt(x32 + i, (VT)((double)ld_nt(x32 + i) + alpha * (double)ld_nt(p + i)));
+    const VT rf = (VT)((double)ld_nt(r32 + i) - alpha * (double)ld_nt(q + i));
+    st_nt(r32 + i, rf);
     rr += (double)rf * (double)rf;
     if (JAC) {
-      const VT wf = d_r[i] * rf;
+      const VT wf = ld_nt(d_r + i) * rf;
       rz += (double)rf * (double)wf;
     }
   }
@@ -447,8 +448,8 @@ __global__ __launch_bounds__(BS) void k_mx_jac_p(int n, const VT *__restrict__ r
   const double beta = s.rz / ctl->rho;
   const size_t stride = (size_t)gridDim.x * BS;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
-    const VT wf = d_r[i] * r32[i];
-    p[i] = (VT)((double)p[i] * beta + (double)wf);
+    const VT wf = ld_nt(d_r + i) * ld_nt(r32 + i);
+    st_nt(p + i, (VT)((double)ld_nt(p + i) * beta + (double)wf));
   }
 }
 
