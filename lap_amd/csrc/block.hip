@@ -347,8 +347,10 @@ __global__ __launch_bounds__(BS) void k_blk_update(int n, int k, int lk, double 
   double rz = 0.0, rr = 0.0;
   if (!ctl->done[col]) {
     for (size_t o = (size_t)blockIdx.x * BS + threadIdx.x; o < total; o += stride) {
-      const double ri = R[o] + nalpha * Q[o];  // r += -alpha*q
-      R[o] = ri;
+      // R and Q streamed once: non-temporal (d_r, shared by a row's k
+      // columns, stays cached)
+      const double ri = ld_nt(R + o) + nalpha * ld_nt(Q + o);  // r += -alpha*q
+      st_nt(R + o, ri);
       rz += ri * (d_r[o >> lk] * ri);
       rr += ri * ri;
     }
@@ -425,9 +427,9 @@ __global__ __launch_bounds__(BS) void k_blk_dir(int n, int k, int lk, double *__
   const double alpha = ctl->alpha[col], beta = ctl->beta[col];
   const size_t total = (size_t)n * k, stride = (size_t)gridDim.x * BS;
   for (size_t o = (size_t)blockIdx.x * BS + threadIdx.x; o < total; o += stride) {
-    const double pv = P[o];
-    if (xup) X[o] += alpha * pv;                       // x += alpha*p
-    if (live) P[o] = pv * beta + d_r[o >> lk] * R[o];  // p *= beta; p += 1.0*w
+    const double pv = ld_nt(P + o);
+    if (xup) st_nt(X + o, ld_nt(X + o) + alpha * pv);                    // x += alpha*p
+    if (live) st_nt(P + o, pv * beta + d_r[o >> lk] * ld_nt(R + o));  // p = beta p + w
   }
 }
 
