@@ -618,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_ch_gather(int n, const int *__restri
   const long long lo = (long long)n * xcd / 8, hi = (long long)n * (xcd + 1) / 8;
   for (long long t = lo + (long long)(blockIdx.x >> 3) * kBlock + threadIdx.x; t < hi;
        t += (long long)per * kBlock)
-    xc[t] = x[crow[t]];
+    st_nt(xc + t, x[ld_nt(crow + t)]);  // the map and xc are streamed; x lines are reused
 }
 
 // The records of the chunked schedule for one set of values (k_tri_chunks).
