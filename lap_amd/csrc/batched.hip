@@ -199,7 +199,7 @@ struct LsPqEpi {
   double acc;
   __device__ double init(int) const { return 0.0; }
   __device__ void store(int row, double v) {
-    q[row] = v;
+    st_nt(q + row, v);  // q is read once, by the update
     acc += p[row] * v;
   }
 };
@@ -347,11 +347,11 @@ __global__ __launch_bounds__(BS) void k_ls_update(int nblk, const int *__restric
     const double alpha = S.alpha[sys], nalpha = -1.0 * alpha;
     const int r0 = rowblk[blk], r1 = rowblk[blk + 1];
     double rz = 0.0, rr = 0.0;
-    for (int i = r0 + (int)threadIdx.x; i < r1; i += BS) {
-      x[i] += alpha * p[i];
-      const double ri = r[i] + nalpha * q[i];
-      r[i] = ri;
-      rz += ri * (d_r[i] * ri);
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += BS) {  // streamed: non-temporal
+      st_nt(x + i, ld_nt(x + i) + alpha * ld_nt(p + i));
+      const double ri = ld_nt(r + i) + nalpha * ld_nt(q + i);
+      st_nt(r + i, ri);
+      rz += ri * (ld_nt(d_r + i) * ri);
       rr += ri * ri;
     }
     const double t_rz = block_sum<BS>(rz, red);
@@ -419,7 +419,8 @@ __global__ __launch_bounds__(BS) void k_ls_p(int nblk, const int *__restrict__ r
     if (ls_skip(blk, bsys, S.done)) continue;
     const double beta = S.beta[bsys[blk]];
     const int r0 = rowblk[blk], r1 = rowblk[blk + 1];
-    for (int i = r0 + (int)threadIdx.x; i < r1; i += BS) p[i] = p[i] * beta + d_r[i] * r[i];
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += BS)
+      st_nt(p + i, ld_nt(p + i) * beta + ld_nt(d_r + i) * ld_nt(r + i));
   }
 }
 
