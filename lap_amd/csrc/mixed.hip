@@ -335,8 +335,9 @@ __global__ __launch_bounds__(BS) void k_mx_update(int n, VT *__restrict__ x32,
   }
 }
 
-// refinement, step 1: x += x32, x32 = 0 (only when this iteration refines);
-// with `xsum` given, partial sums of the new x (for the deflation shift)
+// refinement, step 1 of the generic loop: x += x32, x32 = 0 (only when this
+// iteration refines); with `xsum` given, partial sums of the new x (for the
+// deflation shift).  The Jacobi loop folds in k_mx_jac_p instead.
 template <int BS, class VT>
 __global__ __launch_bounds__(BS) void k_mx_fold(int n, double *__restrict__ x,
                                                 VT *__restrict__ x32, const MixCtl *ctl,
@@ -360,13 +361,16 @@ __global__ __launch_bounds__(BS) void k_mx_fold(int n, double *__restrict__ x,
   }
 }
 
-// refinement, step 2: r32 = (float)(b - A x) in FP64; partials [rz, rr]
+// refinement, step 2: r32 = (float)(b - A x) in FP64; partials [rz, rr].
+// With x32 given (the Jacobi loop, whose fold comes after this kernel, in
+// k_mx_jac_p), the residual is of x + x32: the folded x, bit for bit.
 template <int BS, int TILE, class VT>
 __global__ __launch_bounds__(BS) void k_mx_resid(int nblk, const int *__restrict__ rowblk,
                                                  const int *__restrict__ ia,
                                                  const int *__restrict__ ja,
                                                  const double *__restrict__ a,
                                                  const double *__restrict__ x,
+                                                 const VT *__restrict__ x32,
                                                  const double *__restrict__ rhs, VT *r32,
                                                  const VT *d_r, const MixCtl *ctl,
                                                  const double *upd_part, int upd_cnt,
@@ -377,7 +381,11 @@ __global__ __launch_bounds__(BS) void k_mx_resid(int nblk, const int *__restrict
   const double rr_rec = sum_partials<BS>(upd_part + 1, upd_cnt, 2, red);
   if (!need_refine(rr_rec, ctl->iter + 1, ctl)) return;
   Resid64Epi<VT> epi{rhs, r32, d_r, nullptr, 0.0, 0.0};
-  spmv_walk<BS, TILE>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
+  if (x32)
+    spmv_walk_sum<BS, TILE>(block_walk(nblk, walk), rowblk, ia, ja, a, x, x32, 1.0, prod, red,
+                            epi);
+  else
+    spmv_walk<BS, TILE>(block_walk(nblk, walk), rowblk, ia, ja, a, x, 1.0, prod, red, epi);
   const double t_rz = block_sum<BS>(epi.rz, red);
   const double t_rr = block_sum<BS>(epi.rr, red);
   if (threadIdx.x == 0) {
@@ -417,11 +425,13 @@ __device__ __forceinline__ MxStep mx_step(const MixCtl *ctl, const double *upd_p
   return s;
 }
 
-// Jacobi: convergence bookkeeping, beta, p = beta p + d_r r
+// Jacobi: convergence bookkeeping, the refinement's fold (x += x32, x32 = 0,
+// after k_mx_resid has used x + x32), beta, p = beta p + d_r r
 template <int BS, class VT>
 __global__ __launch_bounds__(BS) void k_mx_jac_p(int n, const VT *__restrict__ r32,
                                                  const VT *__restrict__ d_r,
-                                                 VT *__restrict__ p, MixCtl *ctl,
+                                                 VT *__restrict__ p, double *__restrict__ x,
+                                                 VT *__restrict__ x32, MixCtl *ctl,
                                                  double *hist, const double *upd_part,
                                                  int upd_cnt, const double *ref_part,
                                                  int ref_cnt) {
@@ -444,9 +454,14 @@ __global__ __launch_bounds__(BS) void k_mx_jac_p(int n, const VT *__restrict__ r
       ctl->pend_valid = 1;
     }
   }
+  const size_t stride = (size_t)gridDim.x * BS;
+  if (s.refine)  // also on the last iteration: x is the solution
+    for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
+      x[i] = x[i] + (double)x32[i];
+      x32[i] = VT(0);
+    }
   if (s.flag >= 0) return;
   const double beta = s.rz / ctl->rho;
-  const size_t stride = (size_t)gridDim.x * BS;
   for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < (size_t)n; i += stride) {
     const VT wf = ld_nt(d_r + i) * ld_nt(r32 + i);
     st_nt(p + i, (VT)((double)ld_nt(p + i) * beta + (double)wf));
@@ -976,11 +991,10 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
       spmv32();
       k_mx_update<kBlock, true, VT><<<gV, kBlock, 0, stream()>>>(n, x32, r32, p32, q32, d_r32,
                                                                  ctl, Q1, (int)gO, Q2);
-      k_mx_fold<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, x, x32, ctl, Q2, (int)gV, nullptr);
       k_mx_resid<kBlock, kSpmvTile, VT><<<gA, kBlock, 0, stream()>>>(
-          A64->nblocks, rb_a, ia, ja, a, x, b, r32, d_r32, ctl, Q2, (int)gV, Q3, walk);
-      k_mx_jac_p<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, r32, d_r32, p32, ctl, hist, Q2,
-                                                          (int)gV, Q3, (int)gA);
+          A64->nblocks, rb_a, ia, ja, a, x, x32, b, r32, d_r32, ctl, Q2, (int)gV, Q3, walk);
+      k_mx_jac_p<kBlock, VT><<<gV, kBlock, 0, stream()>>>(n, r32, d_r32, p32, x, x32, ctl, hist,
+                                                          Q2, (int)gV, Q3, (int)gA);
     } else {
       if (ic) {
         ic_inner(*icp, *M, l32, u32, r32, w32, aux32, done);
@@ -1005,11 +1019,12 @@ static int run_mixed(int n, int nnz, const int *ia, const int *ja, const double 
         k_mx_xshift<kBlock><<<gV, kBlock, 0, stream()>>>(n, x, defl_target, ctl, Q2, (int)gV, Q3,
                                                          (int)gV);
       k_mx_resid<kBlock, kSpmvTile, VT><<<gA, kBlock, 0, stream()>>>(
-          A64->nblocks, rb_a, ia, ja, a, x, b, r32, nullptr, ctl, Q2, (int)gV, Q3, walk);
+          A64->nblocks, rb_a, ia, ja, a, x, nullptr, b, r32, nullptr, ctl, Q2, (int)gV, Q3,
+          walk);
       k_mx_conv<kBlock><<<1, kBlock, 0, stream()>>>(ctl, hist, Q2, (int)gV, Q3, (int)gA);
     }
   };
-  int per_iter = jac ? 5 : defl ? 9 : 8;
+  int per_iter = jac ? 4 : defl ? 9 : 8;
   if (ic)
     per_iter += sizeof(VT) == sizeof(double)
                     ? tri_launches(*icp->lplan, &icp->lell, false) +

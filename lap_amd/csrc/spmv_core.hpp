@@ -110,4 +110,43 @@ __device__ __forceinline__ void spmv_walk(BlockWalk w, const int *__restrict__ r
   }
 }
 
+// spmv_walk over x + x2: each gathered value is x[c] + (double)x2[c], the
+// value that folding x2 into x (x[c] += x2[c] in FP64) would have stored, so
+// the sums are those of spmv_walk after such a fold, bit for bit.
+template <int BS, int TILE, class X2T, class Epi>
+__device__ __forceinline__ void spmv_walk_sum(BlockWalk w, const int *__restrict__ rowblk,
+                                              const int *__restrict__ ia,
+                                              const int *__restrict__ ja,
+                                              const double *__restrict__ a,
+                                              const double *__restrict__ x,
+                                              const X2T *__restrict__ x2, double alpha,
+                                              double *prod, double *red, Epi &epi) {
+  for (int s = w.begin; s < w.end; s += w.step) {
+    const int b = w.block(s);
+    if (b >= w.nblk) continue;
+    const int r0 = rowblk[b], r1 = rowblk[b + 1];
+    const int e0 = ia[r0], e1 = ia[r1];
+    const int cnt = e1 - e0;
+    if (cnt > TILE) {  // a long row: as spmv_long_row
+      double t = 0.0;
+#pragma unroll 4
+      for (int e = e0 + (int)threadIdx.x; e < e1; e += BS) {
+        const int c = ja[e];
+        t += (alpha * a[e]) * (x[c] + (double)x2[c]);
+      }
+      const double tt = block_sum<BS>(t, red);
+      if (threadIdx.x == 0) epi.store(r0, epi.init(r0) + tt);
+      continue;
+    }
+#pragma unroll 4
+    for (int k = threadIdx.x; k < cnt; k += BS) {
+      const int c = ja[e0 + k];
+      prod[k] = (alpha * a[e0 + k]) * (x[c] + (double)x2[c]);
+    }
+    __syncthreads();
+    spmv_row_sums<BS>(r0, r1, e0, ia, prod, epi);
+    __syncthreads();
+  }
+}
+
 }  // namespace lap
